@@ -1,0 +1,160 @@
+"""GPU parity: libvrt's HIP kernels against the CPU oracle, bit for bit.
+
+Bar: hit index, face, shadow-evaluated and shadow-occluded bits, primary step
+counts, ray and DDA-step totals and depth-field words bit-exact; RGB8 within
+1 LSB per channel (tolerance written in assert_frames_match)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+RGB_TOL = 1  # LSB per channel
+
+
+def assert_frames_match(g, r, label=""):
+    for k in ("hit", "face", "shadow_eval", "shadow_occ", "steps"):
+        bad = np.argwhere(g[k] != r[k])
+        assert bad.size == 0, "%s %s differs at %d pixels, first %s: gpu %s oracle %s" % (
+            label, k, len(bad), bad[0].tolist(), g[k][tuple(bad[0])], r[k][tuple(bad[0])])
+    if "depth" in g:
+        assert np.array_equal(g["depth"].view(np.uint32), r["depth"].view(np.uint32)), label + " depth differs"
+    d = np.abs(g["rgb"].astype(np.int16) - r["rgb"].astype(np.int16))
+    assert int(d.max()) <= RGB_TOL, "%s rgb differs by %d" % (label, int(d.max()))
+    assert g["rays"] == r["rays"], "%s rays %d vs %d" % (label, g["rays"], r["rays"])
+    assert g["dda_steps"] == r["dda_steps"], "%s steps %d vs %d" % (label, g["dda_steps"], r["dda_steps"])
+
+
+@pytest.mark.parametrize("name", ["tiny", "c1"])
+def test_depth_field_build_matches_oracle(gpu, orc, name):
+    sc = gpu.make_scene(name, seed=3)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    got = ctx.download()
+    ref = orc.depth_field(sc.world, sc.dims)
+    assert np.array_equal(got, ref), "depth field differs at %d voxels" % int((got != ref).sum())
+    ctx.close()
+
+
+@pytest.mark.parametrize("name,seed,mode", [("tiny", 1, 0), ("tiny", 2, 1), ("c1", 1, 0), ("c1", 4, 0),
+                                            ("c1", 5, 1)])
+def test_frame_matches_oracle(gpu, orc, name, seed, mode):
+    print("frame case", name, seed, mode, flush=True)
+    sc = gpu.make_scene(name, seed=seed, depth_view=mode)
+    wdf = orc.depth_field(sc.world, sc.dims)
+    g = gpu.render_on_gpu(sc, aux=True)
+    r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height)
+    assert_frames_match(g, r, "%s/%d/%d" % (name, seed, mode))
+    assert (r["hit"] >= 0).mean() > 0.2  # the frame sees terrain
+
+
+def test_cast_rays_match_oracle(gpu, orc):
+    sc = gpu.make_scene("tiny", seed=7)
+    wdf = orc.depth_field(sc.world, sc.dims)
+    rng = np.random.default_rng(11)
+    X, Y, Z = sc.dims
+    n = 20000
+    start = np.stack([rng.uniform(0, X, n), rng.uniform(30, Y, n), rng.uniform(0, Z, n)], 1).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    # edge cases: zero and signed-zero components, axis-aligned, integer starts,
+    # starts on cell faces, starts outside the grid, zero budget
+    special = []
+    for axis in range(3):
+        for s in (1.0, -1.0):
+            v = np.zeros(3, np.float32)
+            v[axis] = s
+            special.append(([40.5, 45.5, 40.5], v))
+            special.append(([40.0, 45.0, 40.0], v))
+            w = v.copy()
+            w[(axis + 1) % 3] = -0.0
+            special.append(([40.25, 45.75, 40.5], w))
+    special += [([40.5, 45.5, 40.5], [0.6, -0.8, 0.0]), ([48.0, 50.0, 48.0], [0.0, -0.6, 0.8]),
+                ([10.0, 40.0, 10.0], [0.57735026, -0.57735026, 0.57735026]), ([-5.0, 40.0, 10.0], [1.0, 0.0, 0.0]),
+                ([40.5, 63.999, 40.5], [1e-30, -1.0, 1e-30]), ([0.0, 0.0, 0.0], [1.0, 1.0, 1.0])]
+    ss = np.array([s for s, _ in special], np.float32)
+    dd = np.array([v for _, v in special], np.float32)
+    start = np.concatenate([start, ss])
+    d = np.concatenate([d, dd])
+    dist = rng.integers(0, 400, start.shape[0]).astype(np.int32)
+    dist[-len(special):] = 384
+    dist[:50] = 0
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(wdf)
+    g = ctx.cast_rays(start, d, dist, 384)
+    r = orc.cast_rays(wdf, sc.dims, start, d, dist, 384)
+    for k in ("hit", "face", "steps"):
+        bad = np.nonzero(g[k] != r[k])[0]
+        assert bad.size == 0, "%s differs for %d rays, first %d" % (k, bad.size, bad[0])
+    assert np.array_equal(g["hit_pos"].view(np.uint32), r["hit_pos"].view(np.uint32))
+    ctx.close()
+
+
+def test_remove_sphere_sequence_matches_oracle(gpu, orc):
+    sc = gpu.make_scene("c1", seed=9)
+    X, Y, Z = sc.dims
+    ref = orc.depth_field(sc.world, sc.dims)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(ref)
+    rng = np.random.default_rng(5)
+    edits = [(int(rng.integers(0, X)), int(rng.integers(0, 60)), int(rng.integers(0, Z)), int(rng.integers(1, 9)))
+             for _ in range(12)]
+    edits += [(0, 0, 0, 7), (X - 1, Y - 1, Z - 1, 8), (X // 2, 30, Z // 2, 0), (-3, 20, 5, 6)]
+    for (cx, cy, cz, r) in edits:
+        orc.remove_sphere(ref, sc.dims, (cx, cy, cz), r)
+        ctx.remove_sphere(cx, cy, cz, r)
+    got = ctx.download()
+    assert np.array_equal(got, ref), "world differs at %d voxels after edits" % int((got != ref).sum())
+    # the edited world still renders identically
+    g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
+    r = orc.render(ref, sc.dims, sc.frame, sc.width, sc.height)
+    assert_frames_match(g, r, "after edits")
+    ctx.close()
+
+
+def test_place_destroy_and_region_upload(gpu, orc):
+    sc = gpu.make_scene("tiny", seed=4)
+    X, Y, Z = sc.dims
+    ref = orc.depth_field(sc.world, sc.dims)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(ref)
+    for (x, y, z, v) in [(5, 5, 5, 0x123456), (X - 1, 0, Z - 1, 0x00FF00), (X, 0, 0, 7), (-1, 3, 3, 9)]:
+        orc.place_voxel(ref, sc.dims, x, y, z, v)
+        ctx.place_voxel(x, y, z, v)
+    for (x, y, z) in [(6, 6, 6), (0, 0, 0), (X + 4, 1, 1)]:
+        orc.destroy_voxel(ref, sc.dims, x, y, z)
+        ctx.destroy_voxel(x, y, z)
+    assert np.array_equal(ctx.download(), ref)
+    # host edits then a partial upload of the box around them
+    host = ref.copy()
+    for (x, y, z) in [(20, 10, 20), (21, 11, 22), (30, 12, 25)]:
+        host[x + X * y + X * Y * z] = -1
+    ctx.upload_region(host, (15, 5, 15), (35, 20, 30))
+    expect = ref.copy()
+    hv = host.reshape(Z, Y, X)
+    ev = expect.reshape(Z, Y, X)
+    ev[15:30, 5:20, 15:36] = hv[15:30, 5:20, 15:36]
+    assert np.array_equal(ctx.download(), expect)
+    ctx.close()
+
+
+def test_c2_frame_matches_oracle(gpu, orc):
+    """The 1080p drop-in frame: GPU-built depth field checked on a sample of
+    voxels, then the whole frame against the oracle on that world."""
+    sc = gpu.make_scene("c2", seed=1)
+    X, Y, Z = sc.dims
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    rng = np.random.default_rng(2)
+    n = 200000
+    xyz = np.stack([rng.integers(0, X, n), rng.integers(0, Y, n), rng.integers(0, Z, n)], 1).astype(np.int32)
+    # include every voxel of two full columns' neighbourhood slabs near the surface
+    exp = orc.fix_depth_values(sc.world, sc.dims, xyz)
+    idx = xyz[:, 0].astype(np.int64) + X * xyz[:, 1] + X * Y * xyz[:, 2].astype(np.int64)
+    assert np.array_equal(wdf[idx], exp), "depth field differs on %d sampled voxels" % int((wdf[idx] != exp).sum())
+    g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
+    r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height)
+    assert_frames_match(g, r, "c2")
+    ctx.close()

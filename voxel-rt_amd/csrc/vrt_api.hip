@@ -1,0 +1,539 @@
+// vrt_api.hip — C-ABI entry points of libvrt: contexts, uploads, depth field,
+// edits, frame constants and the plain draw.  Each function cites the
+// reference interface it stands in for (src/render.hpp:30-41).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+
+#include "vrt_internal.h"
+
+namespace vrt {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(VRT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+double now_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// computeDepthIndices, src/render.cpp:66-100: offsets with x^2+y^2+z^2 <= 49
+// in z, y, x order; dist = -sqrt(shrunk^2) where each component moves one step
+// toward zero; sqrt of the int is the double overload, stored as float.
+namespace {
+struct TableEntry {
+    float dist;
+    int32_t x, y, z;
+};
+std::vector<TableEntry> reference_table() {
+    const int R = VRT_DEPTH_FIELD_RADIUS;
+    std::vector<TableEntry> t;
+    for (int z = -R; z <= R; z++)
+        for (int y = -R; y <= R; y++)
+            for (int x = -R; x <= R; x++) {
+                if (x * x + y * y + z * z > R * R) continue;
+                const int sx = x - (x > 0) + (x < 0), sy = y - (y > 0) + (y < 0), sz = z - (z > 0) + (z < 0);
+                const float d = (float)(-std::sqrt((double)(sx * sx + sy * sy + sz * sz)));
+                if (d <= (float)R) t.push_back({d, x, y, z});
+            }
+    return t;
+}
+}  // namespace
+
+std::vector<float> depth_table_dist() {
+    std::vector<float> out;
+    for (auto& e : reference_table()) out.push_back(e.dist);
+    return out;
+}
+
+std::vector<int32_t> depth_table_offsets() {
+    std::vector<int32_t> out;
+    for (auto& e : reference_table()) out.insert(out.end(), {e.x, e.y, e.z});
+    return out;
+}
+
+FrameDesc make_frame_desc(const vrt_frame& f) {
+    FrameDesc d;
+    d.f = f;
+    d.render_lim = (float)f.render_dist;
+    return d;
+}
+
+int check_frame(const vrt_frame& f) {
+    if (f.render_dist < 0 || f.render_dist > (1 << 24))
+        return fail(VRT_ERR_INVALID, "render_dist must be in [0, 2^24]");
+    return VRT_OK;
+}
+
+int check_target(const vrt_target& t) {
+    if (t.width <= 0 || t.height <= 0) return fail(VRT_ERR_INVALID, "target width and height must be positive");
+    if ((int64_t)t.width * t.height > ((int64_t)1 << 31)) return fail(VRT_ERR_INVALID, "target too large");
+    if (!t.rgb) return fail(VRT_ERR_INVALID, "target rgb buffer is required");
+    return VRT_OK;
+}
+
+namespace {
+
+int check_ctx(vrt_ctx* c) {
+    if (!c || !c->grid.vox) return fail(VRT_ERR_INVALID, "invalid context");
+    VRT_HIP(hipSetDevice(c->device));
+    return VRT_OK;
+}
+
+// Depth table on the device, sorted by decreasing distance, entries at -6
+// (never counted: the test at src/render.cpp:240 is strict) dropped.
+int upload_table(vrt_ctx* c) {
+    auto t = reference_table();
+    std::vector<int> order(t.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return t[a].dist > t[b].dist; });
+    std::vector<int32_t> packed;
+    std::vector<float> dist;
+    int32_t n_near = 0;
+    for (int i : order) {
+        if (!(t[i].dist > -6.0f)) continue;
+        packed.push_back((t[i].x + 8) | ((t[i].y + 8) << 5) | ((t[i].z + 8) << 10));
+        dist.push_back(t[i].dist);
+        if (t[i].dist > -2.0f) n_near++;
+    }
+    VRT_HIP(hipMalloc(&c->table_packed, packed.size() * sizeof(int32_t)));
+    VRT_HIP(hipMalloc(&c->table_dist, dist.size() * sizeof(float)));
+    VRT_HIP(hipMemcpy(c->table_packed, packed.data(), packed.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    VRT_HIP(hipMemcpy(c->table_dist, dist.data(), dist.size() * sizeof(float), hipMemcpyHostToDevice));
+    c->table.packed = c->table_packed;
+    c->table.dist = c->table_dist;
+    c->table.n = (int32_t)packed.size();
+    c->table.n_near = n_near;
+    return VRT_OK;
+}
+
+void release(vrt_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->own_stream) hipStreamSynchronize(c->own_stream);
+    if (c->grid.vox) hipFree(c->grid.vox);
+    if (c->counters) hipFree(c->counters);
+    if (c->table_packed) hipFree(c->table_packed);
+    if (c->table_dist) hipFree(c->table_dist);
+    if (c->bricks) hipFree(c->bricks);
+    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+// glm's 4x4 helpers on column-major float[16] (src/glm/ext/matrix_transform.inl:18-46,
+// src/glm/detail/type_mat4x4.inl:560-571,630-648, func_geometric.inl:88).
+void mat4_rotate(const float* m, float angle, const float* v, float* out) {
+    const float a = angle;
+    const float c = std::cos(a);
+    const float s = std::sin(a);
+    const float dd = (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2];
+    const float inv = 1.0f / std::sqrt(dd);
+    const float axis[3] = {v[0] * inv, v[1] * inv, v[2] * inv};
+    const float temp[3] = {(1.0f - c) * axis[0], (1.0f - c) * axis[1], (1.0f - c) * axis[2]};
+    float R[3][3];
+    R[0][0] = c + temp[0] * axis[0];
+    R[0][1] = temp[0] * axis[1] + s * axis[2];
+    R[0][2] = temp[0] * axis[2] - s * axis[1];
+    R[1][0] = temp[1] * axis[0] - s * axis[2];
+    R[1][1] = c + temp[1] * axis[1];
+    R[1][2] = temp[1] * axis[2] + s * axis[0];
+    R[2][0] = temp[2] * axis[0] + s * axis[1];
+    R[2][1] = temp[2] * axis[1] - s * axis[0];
+    R[2][2] = c + temp[2] * axis[2];
+    float res[16];
+    for (int j = 0; j < 3; j++)
+        for (int r = 0; r < 4; r++) res[4 * j + r] = (m[r] * R[j][0] + m[4 + r] * R[j][1]) + m[8 + r] * R[j][2];
+    for (int r = 0; r < 4; r++) res[12 + r] = m[12 + r];
+    std::memcpy(out, res, sizeof(res));
+}
+
+void mat4_mul(const float* a, const float* b, float* out) {
+    float res[16];
+    for (int j = 0; j < 4; j++)
+        for (int r = 0; r < 4; r++)
+            res[4 * j + r] = ((a[r] * b[4 * j] + a[4 + r] * b[4 * j + 1]) + a[8 + r] * b[4 * j + 2]) +
+                             a[12 + r] * b[4 * j + 3];
+    std::memcpy(out, res, sizeof(res));
+}
+
+const float kIdentity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+
+}  // namespace
+}  // namespace vrt
+
+using namespace vrt;
+
+extern "C" {
+
+const char* vrt_error_string(void) { return g_error.c_str(); }
+
+const char* vrt_version(void) { return "vrt 0.1 (gfx950)"; }
+
+int vrt_device_count(int32_t* count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return n > 0 ? VRT_OK : fail(VRT_ERR_NODEVICE, "no HIP device visible");
+}
+
+// initRender, src/render.cpp:313-372: grid allocated and filled with -1
+// (:349-351).
+int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
+    if (!out) return fail(VRT_ERR_INVALID, "null out");
+    *out = nullptr;
+    if (X <= 0 || Y <= 0 || Z <= 0 || X > 0x7fffffff || Y > 0x7fffffff || Z > 0x7fffffff)
+        return fail(VRT_ERR_INVALID, "grid dimensions must be in [1, 2^31)");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(VRT_ERR_NODEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(VRT_ERR_INVALID, "device out of range");
+    vrt_ctx* c = new vrt_ctx();
+    c->device = device;
+    const uint64_t count = (uint64_t)X * (uint64_t)Y * (uint64_t)Z;
+    auto bail = [&](int rc) {
+        release(c);
+        return rc;
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipSetDevice"));
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return bail(hip_fail(e, "hipGetDeviceProperties"));
+    c->num_cus = prop.multiProcessorCount;
+    if ((e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamCreate"));
+    c->stream = c->own_stream;
+    if (hipMalloc(&c->grid.vox, count * sizeof(int32_t)) != hipSuccess) {
+        c->grid.vox = nullptr;
+        return bail(fail(VRT_ERR_NOMEM, "cannot allocate the voxel grid"));
+    }
+    c->grid.X = (uint32_t)X;
+    c->grid.Y = (uint32_t)Y;
+    c->grid.Z = (uint32_t)Z;
+    c->grid.XY = (uint64_t)X * (uint64_t)Y;
+    c->grid.count = count;
+    if ((e = hipMalloc(&c->counters, 4 * sizeof(unsigned long long))) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc counters"));
+    const uint64_t nbricks = (uint64_t)((X + 7) / 8) * ((Y + 7) / 8) * ((Z + 7) / 8);
+    if ((e = hipMalloc(&c->bricks, nbricks)) != hipSuccess) return bail(hip_fail(e, "hipMalloc bricks"));
+    int rc = upload_table(c);
+    if (rc) return bail(rc);
+    if ((e = hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned long long), c->stream)) != hipSuccess)
+        return bail(hip_fail(e, "hipMemsetAsync"));
+    rc = launch_fill(c->grid.vox, count, -1, c->stream);
+    if (rc) return bail(rc);
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(hip_fail(e, "hipStreamSynchronize"));
+    vrt_frame_defaults(&c->frame, X, 800, 600);
+    *out = c;
+    return VRT_OK;
+}
+
+int vrt_destroy(vrt_ctx* ctx) {
+    if (!ctx) return fail(VRT_ERR_INVALID, "invalid context");
+    release(ctx);
+    return VRT_OK;
+}
+
+int vrt_set_stream(vrt_ctx* ctx, void* stream) {
+    if (int rc = check_ctx(ctx)) return rc;
+    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    return VRT_OK;
+}
+
+void* vrt_get_stream(vrt_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int vrt_sync(vrt_ctx* ctx) {
+    if (int rc = check_ctx(ctx)) return rc;
+    VRT_HIP(hipStreamSynchronize(ctx->stream));
+    return VRT_OK;
+}
+
+int vrt_grid_device_ptr(vrt_ctx* ctx, int32_t** out) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!out) return fail(VRT_ERR_INVALID, "null out");
+    *out = ctx->grid.vox;
+    return VRT_OK;
+}
+
+int vrt_grid_dims(vrt_ctx* ctx, int64_t dims[3]) {
+    if (int rc = check_ctx(ctx)) return rc;
+    dims[0] = ctx->grid.X;
+    dims[1] = ctx->grid.Y;
+    dims[2] = ctx->grid.Z;
+    return VRT_OK;
+}
+
+// updateGeometry, src/render.cpp:199-202.
+int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!host_voxels) return fail(VRT_ERR_INVALID, "null voxels");
+    VRT_HIP(hipMemcpyAsync(ctx->grid.vox, host_voxels, ctx->grid.count * sizeof(int32_t), hipMemcpyHostToDevice,
+                           ctx->stream));
+    VRT_HIP(hipStreamSynchronize(ctx->stream));
+    return VRT_OK;
+}
+
+int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!device_voxels) return fail(VRT_ERR_INVALID, "null voxels");
+    VRT_HIP(hipMemcpyAsync(ctx->grid.vox, device_voxels, ctx->grid.count * sizeof(int32_t),
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    return VRT_OK;
+}
+
+// updatePartialGeometry, src/render.cpp:204-223: corners swapped when the
+// start voxel's index exceeds the end's, then one x-row of x1-x0+1 words per
+// (y, z) with y in [y0, y1), z in [z0, z1); rows clipped to the grid.
+int vrt_upload_region(vrt_ctx* ctx, const int32_t* host_voxels, int32_t x0, int32_t y0, int32_t z0, int32_t x1,
+                      int32_t y1, int32_t z1) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!host_voxels) return fail(VRT_ERR_INVALID, "null voxels");
+    const GridDesc& g = ctx->grid;
+    auto index = [&](int64_t x, int64_t y, int64_t z) -> int64_t {
+        if (x >= 0 && y >= 0 && z >= 0 && x < g.X && y < g.Y && z < g.Z) return x + (int64_t)g.X * y + (int64_t)g.XY * z;
+        return -1;
+    };
+    if (index(x0, y0, z0) > index(x1, y1, z1)) {
+        std::swap(x0, x1);
+        std::swap(y0, y1);
+        std::swap(z0, z1);
+    }
+    const int64_t xs = std::max<int64_t>(0, x0);
+    const int64_t xe = std::min<int64_t>((int64_t)g.X - 1, x1);  // inclusive end of the row
+    if (xe < xs) return VRT_OK;
+    const size_t row_bytes = (size_t)(xe - xs + 1) * sizeof(int32_t);
+    const int64_t ys = std::max<int64_t>(0, y0), ye = std::min<int64_t>(g.Y, y1);
+    const int64_t zs = std::max<int64_t>(0, z0), ze = std::min<int64_t>(g.Z, z1);
+    if (ye <= ys || ze <= zs) return VRT_OK;
+    // One strided 3D copy instead of (ye-ys)*(ze-zs) row copies.
+    hipMemcpy3DParms p = {};
+    p.srcPtr = make_hipPitchedPtr((void*)host_voxels, g.X * sizeof(int32_t), g.X, g.Y);
+    p.dstPtr = make_hipPitchedPtr(ctx->grid.vox, g.X * sizeof(int32_t), g.X, g.Y);
+    p.srcPos = make_hipPos(xs * sizeof(int32_t), ys, zs);
+    p.dstPos = make_hipPos(xs * sizeof(int32_t), ys, zs);
+    p.extent = make_hipExtent(row_bytes, ye - ys, ze - zs);
+    p.kind = hipMemcpyHostToDevice;
+    VRT_HIP(hipMemcpy3DAsync(&p, ctx->stream));
+    VRT_HIP(hipStreamSynchronize(ctx->stream));
+    return VRT_OK;
+}
+
+int vrt_download(vrt_ctx* ctx, int32_t* host_voxels) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!host_voxels) return fail(VRT_ERR_INVALID, "null voxels");
+    VRT_HIP(hipMemcpyAsync(host_voxels, ctx->grid.vox, ctx->grid.count * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    VRT_HIP(hipStreamSynchronize(ctx->stream));
+    return VRT_OK;
+}
+
+int vrt_depth_table(float* dist, int32_t* off, int32_t cap, int32_t* count) {
+    auto t = reference_table();
+    if (count) *count = (int32_t)t.size();
+    for (int32_t i = 0; i < cap && i < (int32_t)t.size(); i++) {
+        if (dist) dist[i] = t[i].dist;
+        if (off) {
+            off[3 * i] = t[i].x;
+            off[3 * i + 1] = t[i].y;
+            off[3 * i + 2] = t[i].z;
+        }
+    }
+    return VRT_OK;
+}
+
+// computeDepthField threads, src/render.cpp:273-286,354-363.
+int vrt_build_depth_field(vrt_ctx* ctx) {
+    if (int rc = check_ctx(ctx)) return rc;
+    const GridDesc& g = ctx->grid;
+    return launch_fix_depth_box(g, ctx->table, 0, 0, 0, (int32_t)g.X, (int32_t)g.Y, (int32_t)g.Z, 0, 0, 0, -1,
+                                ctx->bricks, ctx->stream);
+}
+
+// fixDepthField, src/render.cpp:226-253, over a box.
+int vrt_fix_depth_field_box(vrt_ctx* ctx, int32_t x0, int32_t y0, int32_t z0, int32_t x1, int32_t y1, int32_t z1) {
+    if (int rc = check_ctx(ctx)) return rc;
+    return launch_fix_depth_box(ctx->grid, ctx->table, x0, y0, z0, x1, y1, z1, 0, 0, 0, -1, nullptr, ctx->stream);
+}
+
+// removeSphere, src/level.cpp:30-56.
+int vrt_remove_sphere(vrt_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t radius) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (radius < -3 || radius > 4096) return fail(VRT_ERR_INVALID, "radius must be in [-3, 4096]");
+    if (int rc = launch_carve_sphere(ctx->grid, cx, cy, cz, radius, ctx->stream)) return rc;
+    const int32_t R = radius + (VRT_DEPTH_FIELD_RADIUS >> 1);
+    if (R <= 0) return VRT_OK;
+    const int64_t x0 = (int64_t)cx - R, x1 = (int64_t)cx + R, y0 = (int64_t)cy - R, y1 = (int64_t)cy + R,
+                  z0 = (int64_t)cz - R, z1 = (int64_t)cz + R;
+    auto c32 = [](int64_t v) { return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v)); };
+    return launch_fix_depth_box(ctx->grid, ctx->table, c32(x0), c32(y0), c32(z0), c32(x1), c32(y1), c32(z1), cx, cy,
+                                cz, R, nullptr, ctx->stream);
+}
+
+// placeVoxel, src/render.cpp:256-262.
+int vrt_place_voxel(vrt_ctx* ctx, int32_t x, int32_t y, int32_t z, int32_t voxel) {
+    if (int rc = check_ctx(ctx)) return rc;
+    const GridDesc& g = ctx->grid;
+    if (x < 0 || y < 0 || z < 0 || (uint32_t)x >= g.X || (uint32_t)y >= g.Y || (uint32_t)z >= g.Z) return VRT_OK;
+    return launch_set_voxel(g, (int64_t)x + (int64_t)g.X * y + (int64_t)g.XY * z, voxel, ctx->stream);
+}
+
+// destroyVoxel, src/render.cpp:265-271.
+int vrt_destroy_voxel(vrt_ctx* ctx, int32_t x, int32_t y, int32_t z) { return vrt_place_voxel(ctx, x, y, z, -1); }
+
+// Start-up globals, src/main.cpp:26-39, src/render.cpp:304-311.
+int vrt_frame_defaults(vrt_frame* f, int64_t grid_width, int32_t width, int32_t height) {
+    if (!f || width <= 0 || height <= 0) return fail(VRT_ERR_INVALID, "invalid frame arguments");
+    std::memset(f, 0, sizeof(*f));
+    f->cam_pos[0] = 195.0f;
+    f->cam_pos[1] = 55.0f;
+    f->cam_pos[2] = 155.0f;
+    f->light_pos[0] = (float)grid_width / 2.0f;
+    f->light_pos[1] = (float)grid_width * 3.0f;
+    f->light_pos[2] = (float)grid_width / 2.0f;
+    std::memcpy(f->rotate_matrix, kIdentity, sizeof(kIdentity));
+    f->aspect_ratio = (float)width / (float)height;
+    f->focal = 1.0f;
+    f->view_depth_field = 0;
+    f->render_dist = 384;
+    return vrt_init_local_lights(f);
+}
+
+// initLocalLights, src/render.cpp:304-311.
+int vrt_init_local_lights(vrt_frame* f) {
+    if (!f) return fail(VRT_ERR_INVALID, "null frame");
+    for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
+        vrt_local_light& L = f->lights[i];
+        L.pos[0] = L.pos[1] = L.pos[2] = -1.0f;
+        L.rgb[0] = L.rgb[1] = L.rgb[2] = 0.0f;
+        L.radius = 64.0f;
+        L.reserved = 0.0f;
+    }
+    return VRT_OK;
+}
+
+// placeLocalLight, src/render.cpp:375-385.
+int vrt_place_local_light(vrt_frame* f, float x, float y, float z, float diffuse) {
+    if (!f) return fail(VRT_ERR_INVALID, "null frame");
+    for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
+        vrt_local_light& L = f->lights[i];
+        if (L.pos[0] < 0 || L.pos[1] < 0 || L.pos[2] < 0) {
+            L.pos[0] = x;
+            L.pos[1] = y;
+            L.pos[2] = z;
+            L.rgb[0] = L.rgb[1] = L.rgb[2] = diffuse;
+            L.radius = 64.0f;
+            return i;
+        }
+    }
+    return -1;
+}
+
+int vrt_mat4_rotate(const float m[16], float angle_rad, const float axis[3], float out[16]) {
+    if (!m || !axis || !out) return fail(VRT_ERR_INVALID, "null argument");
+    mat4_rotate(m, angle_rad, axis, out);
+    return VRT_OK;
+}
+
+int vrt_mat4_mul(const float a[16], const float b[16], float out[16]) {
+    if (!a || !b || !out) return fail(VRT_ERR_INVALID, "null argument");
+    mat4_mul(a, b, out);
+    return VRT_OK;
+}
+
+// doMouseLook, src/controls.cpp:136-142.
+int vrt_camera_matrix(float pitch_rad, float yaw_rad, float out[16]) {
+    if (!out) return fail(VRT_ERR_INVALID, "null argument");
+    const float ax[3] = {1, 0, 0}, ay[3] = {0, 1, 0};
+    float rx[16], ry[16];
+    mat4_rotate(kIdentity, pitch_rad, ax, rx);
+    mat4_rotate(kIdentity, yaw_rad, ay, ry);
+    mat4_mul(ry, rx, out);
+    return VRT_OK;
+}
+
+// lightUpdate, src/render.cpp:400-401: rotate about z by radians(deg), then
+// mat4 * vec4(start, 1) in GLM's pairwise order.
+int vrt_light_orbit(float rotation_deg, const float start[3], float out[3]) {
+    if (!start || !out) return fail(VRT_ERR_INVALID, "null argument");
+    const float az[3] = {0, 0, 1};
+    const float rad = rotation_deg * static_cast<float>(0.01745329251994329576923690768489);
+    float r[16];
+    mat4_rotate(kIdentity, rad, az, r);
+    const float v[4] = {start[0], start[1], start[2], 1.0f};
+    for (int k = 0; k < 3; k++) out[k] = (r[k] * v[0] + r[4 + k] * v[1]) + (r[8 + k] * v[2] + r[12 + k] * v[3]);
+    return VRT_OK;
+}
+
+// updateUniforms, src/render.cpp:289-302.
+int vrt_set_frame(vrt_ctx* ctx, const vrt_frame* f) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!f) return fail(VRT_ERR_INVALID, "null frame");
+    if (int rc = check_frame(*f)) return rc;
+    ctx->frame = *f;
+    ctx->frame_set = true;
+    return VRT_OK;
+}
+
+int vrt_draw_frame(vrt_ctx* ctx, const vrt_frame* f, const vrt_target* t) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!f || !t) return fail(VRT_ERR_INVALID, "null frame or target");
+    if (int rc = check_frame(*f)) return rc;
+    if (int rc = check_target(*t)) return rc;
+    ctx->draws++;
+    return launch_render_plain(ctx->grid, make_frame_desc(*f), *t, ctx->counters, ctx->stream);
+}
+
+// glDrawArrays over the fullscreen quad, src/main.cpp:59.
+int vrt_draw(vrt_ctx* ctx, const vrt_target* t) {
+    if (int rc = check_ctx(ctx)) return rc;
+    return vrt_draw_frame(ctx, &ctx->frame, t);
+}
+
+int vrt_read_stats(vrt_ctx* ctx, vrt_stats* out, int32_t reset) {
+    if (int rc = check_ctx(ctx)) return rc;
+    unsigned long long h[4] = {0, 0, 0, 0};
+    VRT_HIP(hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    VRT_HIP(hipStreamSynchronize(ctx->stream));
+    if (out) {
+        out->rays = h[0];
+        out->dda_steps = h[1];
+        out->pixels = h[2];
+        out->draws = ctx->draws;
+    }
+    if (reset) {
+        VRT_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(h), ctx->stream));
+        VRT_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->draws = 0;
+    }
+    return VRT_OK;
+}
+
+// castRay, src/fshader.glsl:59-129, for a list of rays.
+int vrt_cast_rays(vrt_ctx* ctx, int64_t n, const float* start, const float* dir, const int32_t* dist,
+                  int32_t render_dist, int64_t* hit, float* hit_pos, uint8_t* face, uint32_t* steps) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (n < 0 || (n > 0 && (!start || !dir || !dist))) return fail(VRT_ERR_INVALID, "invalid ray arrays");
+    if (render_dist < 0 || render_dist > (1 << 24)) return fail(VRT_ERR_INVALID, "render_dist must be in [0, 2^24]");
+    return launch_cast_rays(ctx->grid, n, start, dir, dist, render_dist, hit, hit_pos, face, steps, ctx->stream);
+}
+
+int vrt_world_generate_device(vrt_ctx* ctx, const vrt_world_params* p) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!p) return fail(VRT_ERR_INVALID, "null params");
+    return world_generate_device(ctx->grid, *p, ctx->stream);
+}
+
+}  // extern "C"

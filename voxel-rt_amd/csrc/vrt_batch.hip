@@ -1,0 +1,218 @@
+// vrt_batch.hip — batched form: many camera views of uneven resolution in one
+// persistent launch.  A plan uploads the views' descriptors once (largest
+// view first, so the long-running tiles start early and small views fill the
+// tail), then captures "reset the tile queue; launch the persistent kernel"
+// into a hipGraph that every vrt_batch_launch replays.  The queue reset is a
+// one-thread kernel, not a memset node.  The autotuner times every tile shape
+// and queue chunk size and keeps the fastest for the context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "vrt_internal.h"
+
+struct vrt_batch {
+    vrt_ctx* ctx = nullptr;
+    vrt_batch_config cfg{};
+    std::vector<vrt::ViewDesc> views;
+    vrt::ViewDesc* views_dev = nullptr;
+    unsigned int* queue = nullptr;
+    int64_t total_tiles = 0;
+    int32_t workgroups = 0;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace vrt {
+
+namespace {
+
+int check_config(const vrt_batch_config& c) {
+    const bool shape_ok = (c.tile_w == 8 && c.tile_h == 8) || (c.tile_w == 16 && c.tile_h == 4) ||
+                          (c.tile_w == 32 && c.tile_h == 2) || (c.tile_w == 16 && c.tile_h == 16) ||
+                          (c.tile_w == 64 && c.tile_h == 1);
+    if (!shape_ok) return fail(VRT_ERR_INVALID, "tile shape must be 8x8, 16x4, 32x2, 16x16 or 64x1");
+    if (c.chunk_tiles < 1 || c.chunk_tiles > 4096) return fail(VRT_ERR_INVALID, "chunk_tiles must be in [1, 4096]");
+    if (c.waves_per_cu < 0 || c.waves_per_cu > 64) return fail(VRT_ERR_INVALID, "waves_per_cu must be in [0, 64]");
+    return VRT_OK;
+}
+
+void destroy_plan(vrt_batch* b) {
+    if (!b) return;
+    if (b->ctx) hipSetDevice(b->ctx->device);
+    if (b->capture) hipStreamSynchronize(b->capture);
+    if (b->ctx) hipStreamSynchronize(b->ctx->stream);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->views_dev) hipFree(b->views_dev);
+    if (b->queue) hipFree(b->queue);
+    if (b->capture) hipStreamDestroy(b->capture);
+    delete b;
+}
+
+}  // namespace
+
+// Fills descriptors (largest view first) and the tile total.
+int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& cfg, std::vector<ViewDesc>& out,
+                     int64_t& total_tiles) {
+    if (n <= 0 || !views) return fail(VRT_ERR_INVALID, "no views");
+    std::vector<int32_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return (int64_t)views[a].target.width * views[a].target.height >
+               (int64_t)views[b].target.width * views[b].target.height;
+    });
+    out.clear();
+    total_tiles = 0;
+    for (int32_t i : order) {
+        if (int rc = check_frame(views[i].frame)) return rc;
+        if (int rc = check_target(views[i].target)) return rc;
+        ViewDesc d;
+        std::memset(&d, 0, sizeof(d));
+        d.frame = make_frame_desc(views[i].frame);
+        d.target = views[i].target;
+        d.tiles_x = (views[i].target.width + cfg.tile_w - 1) / cfg.tile_w;
+        const int32_t tiles_y = (views[i].target.height + cfg.tile_h - 1) / cfg.tile_h;
+        d.tiles = d.tiles_x * tiles_y;
+        d.first_tile = total_tiles;
+        total_tiles += d.tiles;
+        out.push_back(d);
+    }
+    if (total_tiles > 0xffffffffll - 65536) return fail(VRT_ERR_INVALID, "too many tiles in one batch");
+    return VRT_OK;
+}
+
+int persistent_workgroups(vrt_ctx* c, const vrt_batch_config& cfg, int64_t total_tiles, int32_t* out) {
+    int32_t waves = cfg.waves_per_cu;
+    if (waves == 0) {
+        if (int rc = batch_occupancy(cfg, &waves)) return rc;
+        if (waves <= 0) waves = 8;
+    }
+    int64_t wg = (int64_t)c->num_cus * std::max(1, waves / 4);
+    const int64_t pops = (total_tiles + cfg.chunk_tiles - 1) / cfg.chunk_tiles;
+    const int64_t need = (pops + 3) / 4;  // 4 waves per workgroup, one pop each at least
+    wg = std::max<int64_t>(1, std::min(wg, need));
+    *out = (int32_t)wg;
+    return VRT_OK;
+}
+
+}  // namespace vrt
+
+using namespace vrt;
+
+extern "C" {
+
+int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_batch_config* cfg_in,
+                     vrt_batch** out) {
+    if (!ctx || !ctx->grid.vox || !out) return fail(VRT_ERR_INVALID, "invalid arguments");
+    *out = nullptr;
+    VRT_HIP(hipSetDevice(ctx->device));
+    vrt_batch_config cfg = cfg_in ? *cfg_in : ctx->tuned_cfg;
+    if (int rc = check_config(cfg)) return rc;
+    vrt_batch* b = new vrt_batch();
+    b->ctx = ctx;
+    b->cfg = cfg;
+    auto bail = [&](int rc) {
+        destroy_plan(b);
+        return rc;
+    };
+    if (int rc = build_view_descs(views, n, cfg, b->views, b->total_tiles)) return bail(rc);
+    if (int rc = persistent_workgroups(ctx, cfg, b->total_tiles, &b->workgroups)) return bail(rc);
+    hipError_t e;
+    if ((e = hipMalloc(&b->views_dev, b->views.size() * sizeof(ViewDesc))) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc views"));
+    if ((e = hipMalloc(&b->queue, 64)) != hipSuccess) return bail(hip_fail(e, "hipMalloc queue"));
+    if ((e = hipMemcpy(b->views_dev, b->views.data(), b->views.size() * sizeof(ViewDesc), hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return bail(hip_fail(e, "hipMemcpy views"));
+    if ((e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamBeginCapture"));
+    int rc = launch_queue_reset(b->queue, b->capture);
+    if (!rc)
+        rc = launch_render_batch(ctx->grid, b->views_dev, (int32_t)b->views.size(), b->total_tiles, cfg, b->workgroups,
+                                 b->queue, ctx->counters, b->capture);
+    e = hipStreamEndCapture(b->capture, &b->graph);
+    if (rc) return bail(rc);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamEndCapture"));
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess)
+        return bail(hip_fail(e, "hipGraphInstantiate"));
+    *out = b;
+    return VRT_OK;
+}
+
+int vrt_batch_launch(vrt_batch* b) {
+    if (!b || !b->exec) return fail(VRT_ERR_INVALID, "invalid batch");
+    VRT_HIP(hipSetDevice(b->ctx->device));
+    VRT_HIP(hipGraphLaunch(b->exec, b->ctx->stream));
+    b->ctx->draws += b->views.size();
+    return VRT_OK;
+}
+
+int vrt_batch_destroy(vrt_batch* b) {
+    if (!b) return fail(VRT_ERR_INVALID, "invalid batch");
+    destroy_plan(b);
+    return VRT_OK;
+}
+
+int vrt_draw_views_plain(vrt_ctx* ctx, const vrt_view* views, int32_t n) {
+    if (!views || n < 0) return fail(VRT_ERR_INVALID, "invalid views");
+    for (int32_t i = 0; i < n; i++)
+        if (int rc = vrt_draw_frame(ctx, &views[i].frame, &views[i].target)) return rc;
+    return VRT_OK;
+}
+
+int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, int32_t n, int32_t reps, vrt_batch_config* best,
+                       vrt_tune_entry* results, int32_t max_results, int32_t* n_results) {
+    if (!ctx || !ctx->grid.vox) return fail(VRT_ERR_INVALID, "invalid context");
+    if (reps < 1) reps = 1;
+    VRT_HIP(hipSetDevice(ctx->device));
+    const int32_t shapes[4][2] = {{8, 8}, {16, 4}, {32, 2}, {16, 16}};
+    const int32_t chunks[5] = {1, 2, 4, 8, 16};
+    std::vector<vrt_tune_entry> all;
+    hipEvent_t e0, e1;
+    VRT_HIP(hipEventCreate(&e0));
+    VRT_HIP(hipEventCreate(&e1));
+    int rc = VRT_OK;
+    for (auto& sh : shapes) {
+        for (int32_t ch : chunks) {
+            vrt_batch_config cfg{sh[0], sh[1], ch, 0};
+            vrt_batch* b = nullptr;
+            if ((rc = vrt_batch_create(ctx, views, n, &cfg, &b))) break;
+            std::vector<double> ms;
+            rc = vrt_batch_launch(b);  // warm-up
+            for (int32_t r = 0; r < reps && !rc; r++) {
+                hipEventRecord(e0, ctx->stream);
+                rc = vrt_batch_launch(b);
+                hipEventRecord(e1, ctx->stream);
+                if (hipEventSynchronize(e1) != hipSuccess) rc = fail(VRT_ERR_HIP, "event sync");
+                float t = 0.0f;
+                hipEventElapsedTime(&t, e0, e1);
+                ms.push_back(t);
+            }
+            vrt_batch_destroy(b);
+            if (rc) break;
+            std::sort(ms.begin(), ms.end());
+            all.push_back({cfg, ms[ms.size() / 2]});
+        }
+        if (rc) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    auto it = std::min_element(all.begin(), all.end(),
+                               [](const vrt_tune_entry& a, const vrt_tune_entry& b) { return a.ms < b.ms; });
+    ctx->tuned_cfg = it->config;
+    ctx->tuned = true;
+    if (best) *best = it->config;
+    if (n_results) *n_results = (int32_t)std::min<size_t>(all.size(), (size_t)std::max(0, max_results));
+    for (int32_t i = 0; results && i < max_results && i < (int32_t)all.size(); i++) results[i] = all[i];
+    return VRT_OK;
+}
+
+}  // extern "C"

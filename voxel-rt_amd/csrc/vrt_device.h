@@ -1,0 +1,296 @@
+// vrt_device.h — device-side pieces of the per-pixel path of
+// src/fshader.glsl (castRay :59-129 and main :131-190) for gfx950.
+//
+// Every float operation below is one IEEE binary32 operation in the order
+// the shader states it (the library is built with -ffp-contract=off, correctly
+// rounded division and square root, denormals kept), so the hit index, face,
+// shadow bits and step counts equal a literal scalar restatement of the
+// shader.  The code is arranged for wave64 SIMT: one ray per lane, the axis
+// choice as selects, the voxel index kept incrementally, and the shadow rays
+// of a pixel cast one after another by the same lane.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "vrt_internal.h"
+
+namespace vrt {
+namespace dev {
+
+// GLSL int(float): truncation toward zero; out-of-range saturates, NaN -> 0.
+__device__ __forceinline__ int32_t f2i(float v) {
+    if (v != v) return 0;
+    if (v >= 2147483648.0f) return INT32_MAX;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    return (int32_t)v;
+}
+
+// int(sign(v)) (src/fshader.glsl:71); sign(NaN) = 0.
+__device__ __forceinline__ int32_t isign(float v) { return v > 0.0f ? 1 : (v < 0.0f ? -1 : 0); }
+
+__device__ __forceinline__ float max0(float v) { return v > 0.0f ? v : 0.0f; }
+
+// dot, length, normalize in GLM 1.0.0's order (func_geometric.inl:48-54,88;
+// func_exponential.inl:136-139).
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    float x = ax * bx;
+    float y = ay * by;
+    float z = az * bz;
+    return (x + y) + z;
+}
+
+__device__ __forceinline__ uint8_t unorm8(float c) {
+    float k = c > 0.0f ? (c < 1.0f ? c : 1.0f) : 0.0f;
+    return (uint8_t)(int32_t)(k * 255.0f + 0.5f);
+}
+
+// Result of one castRay.
+struct Hit {
+    int64_t index;   // voxel index or -1
+    float pos[3];    // hitPos (valid on a hit)
+    int32_t axis;    // axis of the last step (-1 if the loop never ran)
+    int32_t step;    // step on that axis
+    uint32_t iters;  // loop iterations (stepCount increments)
+};
+
+// castRay, src/fshader.glsl:59-129.  `lim` = min(float(dist), float(RENDER_DIST)):
+// the loop test `t < dist && t < RENDER_DIST` of :83 is `t < lim` for any t.
+// The voxel index of the current cell is kept incrementally and is only used
+// while the cell is inside the grid (per-axis bounds, :42-45).
+__device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, float sz, float dx, float dy,
+                                         float dz, float lim, Hit& h) {
+    int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
+    const int32_t stx = isign(dx), sty = isign(dy), stz = isign(dz);
+    const int32_t fx = stx > 0, fy = sty > 0, fz = stz > 0;
+    const float ddx = 1.0f / fabsf(dx + 0.000001f);
+    const float ddy = 1.0f / fabsf(dy + 0.000001f);
+    const float ddz = 1.0f / fabsf(dz + 0.000001f);
+    float ix = ((float)(cx + fx) - sx) / dx;
+    float iy = ((float)(cy + fy) - sy) / dy;
+    float iz = ((float)(cz + fz) - sz) / dz;
+
+    // Index deltas of one step on each axis (wrapping; only read in-grid).
+    const uint64_t X = g.X;
+    const uint64_t XY = g.XY;
+    const uint64_t sdx = (uint64_t)(int64_t)stx;
+    const uint64_t sdy = (uint64_t)(int64_t)sty * X;
+    const uint64_t sdz = (uint64_t)(int64_t)stz * XY;
+    uint64_t idx = (uint64_t)(int64_t)cx + X * (uint64_t)(int64_t)cy + XY * (uint64_t)(int64_t)cz;
+
+    float cur = 0.0f;
+    float trav = 0.0f;
+    uint32_t iters = 0;
+    int32_t axis = -1;
+    int64_t result = -1;
+    while (trav < lim) {
+        iters++;
+        trav += 1.0f;
+        const bool bx = (ix < iy) & (ix < iz);
+        const bool by = (iy < ix) & (iy < iz);
+        if (bx) {
+            cur = ix;
+            cx += stx;
+            ix += ddx;
+            idx += sdx;
+            axis = 0;
+        } else if (by) {
+            cur = iy;
+            cy += sty;
+            iy += ddy;
+            idx += sdy;
+            axis = 1;
+        } else {
+            cur = iz;
+            cz += stz;
+            iz += ddz;
+            idx += sdz;
+            axis = 2;
+        }
+        const bool inb = ((uint32_t)cx < g.X) & ((uint32_t)cy < g.Y) & ((uint32_t)cz < g.Z);
+        if (!inb) break;
+        const int32_t v = g.vox[idx];
+        if (v >= 0) {
+            h.pos[0] = dx * cur + sx;
+            h.pos[1] = dy * cur + sy;
+            h.pos[2] = dz * cur + sz;
+            result = (int64_t)idx;
+            break;
+        }
+        if (v != -1) {
+            const float toJump = -__int_as_float(v);
+            trav += toJump;
+            cur += toJump;
+            sx = dx * cur + sx;
+            sy = dy * cur + sy;
+            sz = dz * cur + sz;
+            cx = f2i(sx);
+            cy = f2i(sy);
+            cz = f2i(sz);
+            ix = ((float)(cx + fx) - sx) / dx;
+            iy = ((float)(cy + fy) - sy) / dy;
+            iz = ((float)(cz + fz) - sz) / dz;
+            idx = (uint64_t)(int64_t)cx + X * (uint64_t)(int64_t)cy + XY * (uint64_t)(int64_t)cz;
+        }
+    }
+    h.index = result;
+    h.axis = axis;
+    h.step = axis == 0 ? stx : (axis == 1 ? sty : stz);
+    h.iters = iters;
+}
+
+// Per-pixel outputs kept in registers until the stores.
+struct PixelResult {
+    uint8_t rgb[3];
+    float depth;
+    int64_t hit;
+    uint8_t face;
+    uint32_t eval, occ;
+    uint32_t steps;
+    uint32_t rays;
+    uint32_t iters;
+};
+
+// main(), src/fshader.glsl:131-190, for the fragment at vPos = (vx, vy).
+__device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& F, float vx, float vy,
+                                            PixelResult& o) {
+    const vrt_frame& f = F.f;
+    float col0 = 0.6f, col1 = 0.7f, col2 = 0.8f;
+
+    // normalize(vec3(vPos.x*aspectRatio, vPos.y, focal)) (:136)
+    const float ux = vx * f.aspect_ratio, uy = vy, uz = f.focal;
+    const float un = 1.0f / sqrtf(dot3(ux, uy, uz, ux, uy, uz));
+    const float rx = ux * un, ry = uy * un, rz = uz * un;
+    // vec3(rotateMatrix * vec4(d, 0)) (:137), GLM's pairwise column sum
+    const float* M = f.rotate_matrix;
+    const float d0 = (M[0] * rx + M[4] * ry) + (M[8] * rz + M[12] * 0.0f);
+    const float d1 = (M[1] * rx + M[5] * ry) + (M[9] * rz + M[13] * 0.0f);
+    const float d2 = (M[2] * rx + M[6] * ry) + (M[10] * rz + M[14] * 0.0f);
+
+    Hit h;
+    cast_ray(g, f.cam_pos[0], f.cam_pos[1], f.cam_pos[2], d0, d1, d2, F.render_lim, h);
+    uint32_t iters = h.iters;
+    uint32_t rays = 1;
+    uint32_t eval = 0, occ = 0;
+    const bool hit = h.index != -1;
+    const float hx = hit ? h.pos[0] : 0.0f, hy = hit ? h.pos[1] : 0.0f, hz = hit ? h.pos[2] : 0.0f;
+    // firstHitNormal = -step on the axis of the last step
+    const float nx = h.axis == 0 ? (float)(-h.step) : 0.0f;
+    const float ny = h.axis == 1 ? (float)(-h.step) : 0.0f;
+    const float nz = h.axis == 2 ? (float)(-h.step) : 0.0f;
+
+    o.hit = h.index;
+    o.face = hit ? (uint8_t)(h.axis * 3 + (h.step + 1)) : (uint8_t)VRT_FACE_MISS;
+    o.steps = h.iters;
+    if (hit) {
+        const float ex = hx - f.cam_pos[0], ey = hy - f.cam_pos[1], ez = hz - f.cam_pos[2];
+        o.depth = sqrtf(dot3(ex, ey, ez, ex, ey, ez));
+    } else {
+        o.depth = __builtin_inff();
+    }
+
+    if (f.view_depth_field == 1) {
+        const float s = (float)h.iters / 100.0f;
+        col0 = s;
+        col1 = s;
+        col2 = s;
+    } else if (hit) {
+        const int32_t vcol = g.vox[h.index];
+        // toLight = normalize(lightPos - firstHitPos) (:147)
+        const float lx0 = f.light_pos[0] - hx, ly0 = f.light_pos[1] - hy, lz0 = f.light_pos[2] - hz;
+        const float ln = 1.0f / sqrtf(dot3(lx0, ly0, lz0, lx0, ly0, lz0));
+        const float tlx = lx0 * ln, tly = ly0 * ln, tlz = lz0 * ln;
+        float m0 = 0.4f, m1 = 0.4f, m2 = 0.4f;  // AMBIENT
+
+        // sun shadow (:152-156)
+        {
+            Hit s;
+            cast_ray(g, hx + tlx * 0.001f, hy + tly * 0.001f, hz + tlz * 0.001f, tlx, tly, tlz, F.render_lim, s);
+            iters += s.iters;
+            rays++;
+            eval |= 1u;
+            if (s.index == -1) {
+                const float add = 0.8f * max0(dot3(nx, ny, nz, tlx, tly, tlz));  // DIFFUSE
+                m0 += add;
+                m1 += add;
+                m2 += add;
+            } else {
+                occ |= 1u;
+            }
+        }
+
+        // local lights (:158-181)
+        for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
+            const vrt_local_light& L = f.lights[i];
+            if ((m0 >= 1.25f) & (m1 >= 1.25f) & (m2 >= 1.25f)) {  // MAX_OVERBRIGHT
+                m0 = 1.25f;
+                m1 = 1.25f;
+                m2 = 1.25f;
+                break;
+            }
+            if (!((L.pos[0] >= 0.0f) & (L.pos[1] >= 0.0f) & (L.pos[2] >= 0.0f))) continue;
+            const float vx0 = L.pos[0] - hx, vy0 = L.pos[1] - hy, vz0 = L.pos[2] - hz;
+            const float dist = sqrtf(dot3(vx0, vy0, vz0, vx0, vy0, vz0));
+            if (!(dist <= L.radius)) continue;
+            const float inv = 1.0f / dist;
+            const float ux1 = vx0 * inv, uy1 = vy0 * inv, uz1 = vz0 * inv;
+            float lim = (float)f2i(dist + 1.0f);
+            lim = lim < F.render_lim ? lim : F.render_lim;
+            Hit s;
+            cast_ray(g, hx + ux1 * 0.001f, hy + uy1 * 0.001f, hz + uz1 * 0.001f, ux1, uy1, uz1, lim, s);
+            iters += s.iters;
+            rays++;
+            eval |= 2u << i;
+            if (s.index == -1) {
+                const float lam = max0(dot3(nx, ny, nz, ux1, uy1, uz1));
+                const float fall = (L.radius - dist) / L.radius;
+                m0 += L.rgb[0] * lam * fall;
+                m1 += L.rgb[1] * lam * fall;
+                m2 += L.rgb[2] * lam * fall;
+            } else {
+                occ |= 2u << i;
+            }
+        }
+
+        col0 = (float)((vcol & 0x00FF0000) >> 16) / 255.0f * m0;
+        col1 = (float)((vcol & 0x0000FF00) >> 8) / 255.0f * m1;
+        col2 = (float)(vcol & 0x000000FF) / 255.0f * m2;
+    }
+    o.rgb[0] = unorm8(col0);
+    o.rgb[1] = unorm8(col1);
+    o.rgb[2] = unorm8(col2);
+    o.eval = eval;
+    o.occ = occ;
+    o.rays = rays;
+    o.iters = iters;
+}
+
+__device__ __forceinline__ void store_pixel(const vrt_target& t, int64_t k, const PixelResult& p) {
+    uint8_t* c = t.rgb + 3 * k;
+    c[0] = p.rgb[0];
+    c[1] = p.rgb[1];
+    c[2] = p.rgb[2];
+    if (t.depth) t.depth[k] = p.depth;
+    if (t.hit) t.hit[k] = p.hit;
+    if (t.face) t.face[k] = p.face;
+    if (t.shadow_eval) t.shadow_eval[k] = p.eval;
+    if (t.shadow_occ) t.shadow_occ[k] = p.occ;
+    if (t.steps) t.steps[k] = p.steps;
+}
+
+// vPos of pixel (px, row), row 0 at the top (GL rows count from the bottom).
+__device__ __forceinline__ void pixel_vpos(int32_t W, int32_t H, int32_t px, int32_t row, float& vx, float& vy) {
+    const int32_t py = H - 1 - row;
+    vx = (float)(2 * px + 1) / (float)W - 1.0f;
+    vy = (float)(2 * py + 1) / (float)H - 1.0f;
+}
+
+// Sum over the wave, result in every lane.
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace dev
+}  // namespace vrt

@@ -1,0 +1,191 @@
+// vrt_edit.hip — depth field and voxel edits on the device.
+//
+// fixDepthField (src/render.cpp:226-253) gives an empty voxel (word < 0) the
+// largest table distance d* over its SOLID neighbours with d* > -6 (strict,
+// :240), and then: d* > -2 -> the word is left as it was (:244-249);
+// -6 < d* <= -2 -> the bits of d*; no such neighbour -> the bits of -6.0f.
+// The result depends only on which neighbours are solid, and a jump word is
+// negative (never solid), so every voxel of a region can be fixed in
+// parallel, in place, with the same result as the reference's sequential
+// loops.  The table is scanned in order of decreasing distance, so the first
+// solid neighbour found is d*.  For whole-grid builds a workgroup owns one
+// 8x8x8 brick and first checks the 3x3x3 neighbouring bricks' solid flags:
+// with none solid, every empty voxel of the brick takes -6 without a scan.
+#include <hip/hip_runtime.h>
+
+#include "vrt_internal.h"
+
+namespace vrt {
+
+namespace {
+
+constexpr int32_t kMinus6Bits = (int32_t)0xC0C00000;  // bits of -6.0f
+
+__device__ __forceinline__ int32_t fix_value(const GridDesc& g, const DepthTableDev& t, int32_t x, int32_t y,
+                                             int32_t z, int32_t old) {
+    for (int32_t k = 0; k < t.n; k++) {
+        const int32_t p = t.packed[k];
+        const int32_t nx = x + (p & 31) - 8;
+        const int32_t ny = y + ((p >> 5) & 31) - 8;
+        const int32_t nz = z + ((p >> 10) & 31) - 8;
+        if (((uint32_t)nx < g.X) & ((uint32_t)ny < g.Y) & ((uint32_t)nz < g.Z)) {
+            const uint64_t i = (uint64_t)nx + (uint64_t)g.X * (uint64_t)ny + g.XY * (uint64_t)nz;
+            if (g.vox[i] >= 0) return k < t.n_near ? old : __float_as_int(t.dist[k]);
+        }
+    }
+    return kMinus6Bits;
+}
+
+// Any solid voxel in each 8x8x8 brick.
+__global__ void __launch_bounds__(256) k_brick_flags(GridDesc g, uint8_t* flags, uint32_t bx, uint32_t by,
+                                                      uint32_t bz) {
+    const uint64_t b = blockIdx.x;
+    const uint32_t ix = (uint32_t)(b % bx), iy = (uint32_t)((b / bx) % by), iz = (uint32_t)(b / ((uint64_t)bx * by));
+    int any = 0;
+    for (int c = threadIdx.x; c < 512; c += 256) {
+        const uint32_t x = ix * 8 + (c & 7), y = iy * 8 + ((c >> 3) & 7), z = iz * 8 + (c >> 6);
+        if (x < g.X && y < g.Y && z < g.Z) any |= g.vox[(uint64_t)x + g.X * (uint64_t)y + g.XY * z] >= 0;
+    }
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) flags[b] = (uint8_t)(any != 0);
+}
+
+__global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTableDev t, const uint8_t* flags,
+                                                           uint32_t bx, uint32_t by, uint32_t bz) {
+    const uint64_t b = blockIdx.x;
+    const int32_t ix = (int32_t)(b % bx), iy = (int32_t)((b / bx) % by), iz = (int32_t)(b / ((uint64_t)bx * by));
+    bool near_solid = false;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int32_t jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                if (jx >= 0 && jy >= 0 && jz >= 0 && jx < (int32_t)bx && jy < (int32_t)by && jz < (int32_t)bz)
+                    near_solid |= flags[(uint64_t)jx + (uint64_t)bx * jy + (uint64_t)bx * by * jz] != 0;
+            }
+    const int32_t c = threadIdx.x;
+    const int32_t x = ix * 8 + (c & 7), y = iy * 8 + ((c >> 3) & 7), z = iz * 8 + (c >> 6);
+    if ((uint32_t)x >= g.X || (uint32_t)y >= g.Y || (uint32_t)z >= g.Z) return;
+    const uint64_t i = (uint64_t)x + g.X * (uint64_t)y + g.XY * (uint64_t)z;
+    const int32_t old = g.vox[i];
+    if (old >= 0) return;
+    // A solid neighbour within radius 7 lies in the 3x3x3 bricks around this one.
+    const int32_t nv = near_solid ? fix_value(g, t, x, y, z, old) : kMinus6Bits;
+    if (nv != old) g.vox[i] = nv;
+}
+
+// fixDepthField over the box [x0,x1) x [y0,y1) x [z0,z1) (already clipped);
+// with sr >= 0 only voxels with (x-cx)^2+(y-cy)^2+(z-cz)^2 < sr^2.
+__global__ void __launch_bounds__(256) k_fix_depth_box(GridDesc g, DepthTableDev t, int32_t x0, int32_t y0,
+                                                        int32_t z0, int32_t w, int32_t h, int32_t d, int32_t cx,
+                                                        int32_t cy, int32_t cz, int32_t sr) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)w * h * d) return;
+    const int32_t x = x0 + (int32_t)(k % w);
+    const int32_t y = y0 + (int32_t)((k / w) % h);
+    const int32_t z = z0 + (int32_t)(k / ((int64_t)w * h));
+    if (sr >= 0) {
+        const int32_t ex = x - cx, ey = y - cy, ez = z - cz;
+        if (ex * ex + ey * ey + ez * ez >= sr * sr) return;
+    }
+    const uint64_t i = (uint64_t)x + g.X * (uint64_t)y + g.XY * (uint64_t)z;
+    const int32_t old = g.vox[i];
+    if (old >= 0) return;
+    const int32_t nv = fix_value(g, t, x, y, z, old);
+    if (nv != old) g.vox[i] = nv;
+}
+
+// destroyVoxel over the clipped box of removeSphere's first loop
+// (src/level.cpp:31-42): (x-cx)^2+(y-cy)^2+(z-cz)^2 < r^2.
+__global__ void __launch_bounds__(256) k_carve(GridDesc g, int32_t x0, int32_t y0, int32_t z0, int32_t w, int32_t h,
+                                                int32_t d, int32_t cx, int32_t cy, int32_t cz, int32_t r) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)w * h * d) return;
+    const int32_t x = x0 + (int32_t)(k % w);
+    const int32_t y = y0 + (int32_t)((k / w) % h);
+    const int32_t z = z0 + (int32_t)(k / ((int64_t)w * h));
+    const int32_t ex = x - cx, ey = y - cy, ez = z - cz;
+    if (ex * ex + ey * ey + ez * ez >= r * r) return;
+    g.vox[(uint64_t)x + g.X * (uint64_t)y + g.XY * (uint64_t)z] = -1;
+}
+
+__global__ void k_set_voxel(int32_t* vox, uint64_t i, int32_t v) {
+    if (threadIdx.x == 0) vox[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill(int32_t* p, uint64_t n, int32_t v) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Clip [lo, hi) to [0, n).
+inline void clip(int64_t& lo, int64_t& hi, int64_t n) {
+    if (lo < 0) lo = 0;
+    if (hi > n) hi = n;
+}
+
+}  // namespace
+
+int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, int32_t y0, int32_t z0, int32_t x1,
+                         int32_t y1, int32_t z1, int32_t scx, int32_t scy, int32_t scz, int32_t sr,
+                         uint8_t* brick_scratch, hipStream_t s) {
+    int64_t ax = x0, bx_ = x1, ay = y0, by_ = y1, az = z0, bz_ = z1;
+    clip(ax, bx_, g.X);
+    clip(ay, by_, g.Y);
+    clip(az, bz_, g.Z);
+    if (bx_ <= ax || by_ <= ay || bz_ <= az) return VRT_OK;
+    const bool whole = ax == 0 && ay == 0 && az == 0 && bx_ == g.X && by_ == g.Y && bz_ == g.Z && sr < 0;
+    if (whole && brick_scratch) {
+        const uint32_t nbx = (g.X + 7) / 8, nby = (g.Y + 7) / 8, nbz = (g.Z + 7) / 8;
+        const uint64_t nb = (uint64_t)nbx * nby * nbz;
+        if (nb > 0x7fffffffull) return fail(VRT_ERR_INVALID, "grid too large for brick build");
+        hipLaunchKernelGGL(k_brick_flags, dim3((uint32_t)nb), dim3(256), 0, s, g, brick_scratch, nbx, nby, nbz);
+        VRT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_fix_depth_bricks, dim3((uint32_t)nb), dim3(512), 0, s, g, t, brick_scratch, nbx, nby,
+                           nbz);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
+    const int64_t w = bx_ - ax, h = by_ - ay, d = bz_ - az;
+    const int64_t n = w * h * d;
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "box too large");
+    hipLaunchKernelGGL(k_fix_depth_box, dim3((uint32_t)blocks), dim3(256), 0, s, g, t, (int32_t)ax, (int32_t)ay,
+                       (int32_t)az, (int32_t)w, (int32_t)h, (int32_t)d, scx, scy, scz, sr);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_carve_sphere(const GridDesc& g, int32_t cx, int32_t cy, int32_t cz, int32_t r, hipStream_t s) {
+    if (r <= 0) return VRT_OK;
+    int64_t ax = (int64_t)cx - r, bx_ = (int64_t)cx + r, ay = (int64_t)cy - r, by_ = (int64_t)cy + r,
+            az = (int64_t)cz - r, bz_ = (int64_t)cz + r;
+    clip(ax, bx_, g.X);
+    clip(ay, by_, g.Y);
+    clip(az, bz_, g.Z);
+    if (bx_ <= ax || by_ <= ay || bz_ <= az) return VRT_OK;
+    const int64_t w = bx_ - ax, h = by_ - ay, d = bz_ - az;
+    const int64_t blocks = (w * h * d + 255) / 256;
+    if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "sphere too large");
+    hipLaunchKernelGGL(k_carve, dim3((uint32_t)blocks), dim3(256), 0, s, g, (int32_t)ax, (int32_t)ay, (int32_t)az,
+                       (int32_t)w, (int32_t)h, (int32_t)d, cx, cy, cz, r);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_set_voxel(const GridDesc& g, int64_t index, int32_t v, hipStream_t s) {
+    if (index < 0 || (uint64_t)index >= g.count) return fail(VRT_ERR_INVALID, "voxel index out of range");
+    hipLaunchKernelGGL(k_set_voxel, dim3(1), dim3(64), 0, s, g.vox, (uint64_t)index, v);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_fill(int32_t* p, uint64_t n, int32_t v, hipStream_t s) {
+    if (n == 0) return VRT_OK;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536ull * 16) blocks = 65536ull * 16;
+    hipLaunchKernelGGL(k_fill, dim3((uint32_t)blocks), dim3(256), 0, s, p, n, v);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+}  // namespace vrt

@@ -1,0 +1,118 @@
+// vrt_internal.h — declarations shared by the libvrt sources (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vrt.h"
+
+namespace vrt {
+
+// Sets the thread's last error message and returns `code`.
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define VRT_HIP(call)                                        \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return vrt::hip_fail(e_, #call); \
+    } while (0)
+
+// Grid as the kernels see it.
+struct GridDesc {
+    int32_t* vox;
+    uint32_t X, Y, Z;
+    uint64_t XY;
+    uint64_t count;
+};
+
+// One frame as the kernels see it: vrt_frame plus derived per-frame values.
+struct FrameDesc {
+    vrt_frame f;
+    float render_lim;   // (float)render_dist
+};
+
+// Per-view work description for the persistent batch kernel.
+struct ViewDesc {
+    FrameDesc frame;
+    vrt_target target;
+    int32_t tiles_x;       // tiles across
+    int32_t tiles;         // tiles in this view
+    int64_t first_tile;    // prefix sum of tiles over earlier views
+};
+
+struct DepthTableDev {
+    const int32_t* packed;  // (dx+8) | (dy+8)<<5 | (dz+8)<<10 | group<<15, sorted by dist descending
+    const float* dist;      // dist of each packed entry
+    int32_t n;              // entries with dist > -6
+    int32_t n_near;         // leading entries with dist > -2
+};
+
+struct Ctx;
+
+// ---- kernels launch wrappers (vrt_render.hip / vrt_edit.hip / vrt_world.hip)
+int launch_render_plain(const GridDesc& g, const FrameDesc& f, const vrt_target& t,
+                        unsigned long long* counters, hipStream_t s);
+int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const float* dir,
+                     const int32_t* dist, int32_t render_dist, int64_t* hit, float* hit_pos,
+                     uint8_t* face, uint32_t* steps, hipStream_t s);
+int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_views,
+                        int64_t total_tiles, const vrt_batch_config& cfg, int32_t n_workgroups,
+                        unsigned int* queue, unsigned long long* counters, hipStream_t s);
+int launch_queue_reset(unsigned int* queue, hipStream_t s);
+int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu);
+
+int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, int32_t y0, int32_t z0,
+                         int32_t x1, int32_t y1, int32_t z1, int32_t sphere_cx, int32_t sphere_cy,
+                         int32_t sphere_cz, int32_t sphere_r, uint8_t* brick_scratch, hipStream_t s);
+int launch_carve_sphere(const GridDesc& g, int32_t cx, int32_t cy, int32_t cz, int32_t r, hipStream_t s);
+int launch_set_voxel(const GridDesc& g, int64_t index, int32_t v, hipStream_t s);
+int launch_fill(int32_t* p, uint64_t n, int32_t v, hipStream_t s);
+int launch_world_generate(const GridDesc& g, const vrt_world_params& p, const int32_t* cave_list,
+                          int32_t n_caves, const int32_t* box_list, int32_t n_boxes,
+                          const int32_t* cave_bins, const int32_t* box_bins, int32_t bin_size,
+                          int32_t bins_x, int32_t bins_z, hipStream_t s);
+
+// ---- host helpers
+std::vector<float> depth_table_dist();              // reference order
+std::vector<int32_t> depth_table_offsets();         // reference order, xyz triples
+double now_seconds();
+
+// World generator pieces shared by the host and device generators.
+struct WorldPlan {
+    vrt_world_params p;
+    std::vector<int32_t> caves;  // cx, cy, cz, r
+    std::vector<int32_t> boxes;  // x0, y0, z0, x1, y1, z1 (half-open, clipped)
+};
+int make_world_plan(const vrt_world_params& p, WorldPlan& plan);
+
+}  // namespace vrt
+
+// Context behind the opaque vrt_ctx handle.
+struct vrt_ctx {
+    int32_t device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    vrt::GridDesc grid{};
+    vrt_frame frame{};
+    bool frame_set = false;
+    unsigned long long* counters = nullptr;  // rays, iters, pixels
+    uint64_t draws = 0;
+    int32_t* table_packed = nullptr;
+    float* table_dist = nullptr;
+    vrt::DepthTableDev table{};
+    uint8_t* bricks = nullptr;
+    int32_t num_cus = 0;
+    bool tuned = false;
+    vrt_batch_config tuned_cfg{8, 8, 4, 0};
+};
+
+namespace vrt {
+int world_generate_device(const GridDesc& g, const vrt_world_params& p, hipStream_t s);
+FrameDesc make_frame_desc(const vrt_frame& f);
+int check_frame(const vrt_frame& f);
+int check_target(const vrt_target& t);
+}  // namespace vrt

@@ -1,0 +1,200 @@
+// vrt_render.hip — render kernels for gfx950 (wave64).
+//
+//   k_render_plain : one frame; a wave shades one 8x8 screen tile (one ray per
+//                    lane), a 256-thread workgroup four horizontally adjacent
+//                    tiles; workgroups are renumbered so that each of the 8
+//                    XCDs (round-robin dispatch) walks a contiguous band of
+//                    the image and its L2 sees neighbouring rays.
+//   k_render_batch : many views in one persistent launch; every wave pops
+//                    chunks of (view, tile) work from a device atomic queue
+//                    until the queue is empty (the exit every wave reaches).
+//   k_cast_rays    : castRay for an arbitrary list of rays.
+// The per-pixel work is src/fshader.glsl main() (vrt_device.h).
+#include <hip/hip_runtime.h>
+
+#include "vrt_device.h"
+#include "vrt_internal.h"
+
+namespace vrt {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kPlainWaves = 4;                 // waves per workgroup
+constexpr int kPlainThreads = kWave * kPlainWaves;
+constexpr int kXcds = 8;
+
+// Bijection from dispatch order to a logical workgroup id such that the
+// workgroups landing on XCD k (dispatch ids = k mod 8) own a contiguous range.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
+    const uint32_t xcd = bid % kXcds, local = bid / kXcds;
+    const uint32_t base = nb / kXcds, rem = nb % kXcds;
+    const uint32_t start = xcd * base + (xcd < rem ? xcd : rem);
+    return start + local;
+}
+
+__device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long rays,
+                                               unsigned long long iters, unsigned long long pixels) {
+    if (!counters) return;
+    rays = dev::wave_sum(rays);
+    iters = dev::wave_sum(iters);
+    pixels = dev::wave_sum(pixels);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        atomicAdd(counters + 0, rays);
+        atomicAdd(counters + 1, iters);
+        atomicAdd(counters + 2, pixels);
+    }
+}
+
+__global__ void __launch_bounds__(kPlainThreads) k_render_plain(GridDesc g, FrameDesc F, vrt_target t,
+                                                                unsigned long long* counters) {
+    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int32_t tiles_x = (t.width + 7) >> 3;
+    const int32_t tiles_y = (t.height + 7) >> 3;
+    const int32_t tile = (int32_t)lb * kPlainWaves + (int32_t)(threadIdx.x >> 6);
+    const int32_t lane = threadIdx.x & 63;
+    unsigned long long rays = 0, iters = 0, pixels = 0;
+    if (tile < tiles_x * tiles_y) {
+        const int32_t px = (tile % tiles_x) * 8 + (lane & 7);
+        const int32_t row = (tile / tiles_x) * 8 + (lane >> 3);
+        if (px < t.width && row < t.height) {
+            float vx, vy;
+            dev::pixel_vpos(t.width, t.height, px, row, vx, vy);
+            dev::PixelResult p;
+            dev::shade_pixel(g, F, vx, vy, p);
+            dev::store_pixel(t, (int64_t)row * t.width + px, p);
+            rays = p.rays;
+            iters = p.iters;
+            pixels = 1;
+        }
+    }
+    flush_counters(counters, rays, iters, pixels);
+}
+
+__global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const float* __restrict__ start,
+                                                   const float* __restrict__ dir,
+                                                   const int32_t* __restrict__ dist, float render_lim,
+                                                   int64_t* hit, float* hit_pos, uint8_t* face, uint32_t* steps) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float lim = (float)dist[i];
+    lim = lim < render_lim ? lim : render_lim;
+    dev::Hit h;
+    dev::cast_ray(g, start[3 * i], start[3 * i + 1], start[3 * i + 2], dir[3 * i], dir[3 * i + 1], dir[3 * i + 2],
+                  lim, h);
+    const bool hitp = h.index != -1;
+    if (hit) hit[i] = h.index;
+    if (hit_pos) {
+        hit_pos[3 * i] = hitp ? h.pos[0] : 0.0f;
+        hit_pos[3 * i + 1] = hitp ? h.pos[1] : 0.0f;
+        hit_pos[3 * i + 2] = hitp ? h.pos[2] : 0.0f;
+    }
+    if (face) face[i] = hitp ? (uint8_t)(h.axis * 3 + (h.step + 1)) : (uint8_t)VRT_FACE_MISS;
+    if (steps) steps[i] = h.iters;
+}
+
+// Persistent multi-view kernel.  Tile shape tw x th with tw*th a multiple of
+// 64: a wave shades 64 pixels per pass (lane -> (lane % tw, lane / tw)) and
+// tw*th/64 passes per tile.  The queue counts tiles; a pop takes `chunk`.
+__global__ void __launch_bounds__(256) k_render_batch(GridDesc g, const ViewDesc* __restrict__ views,
+                                                      int32_t n_views, int64_t total_tiles, int32_t tw,
+                                                      int32_t th, int32_t chunk, unsigned int* queue,
+                                                      unsigned long long* counters) {
+    const int32_t lane = threadIdx.x & 63;
+    const int32_t rows_per_pass = kWave / tw;
+    const int32_t passes = th / rows_per_pass;
+    unsigned long long rays = 0, iters = 0, pixels = 0;
+    int32_t v = 0;  // views are popped in increasing order: search forward
+    for (;;) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(queue, (unsigned int)chunk);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if ((int64_t)base >= total_tiles) break;
+        int64_t end = (int64_t)base + chunk;
+        if (end > total_tiles) end = total_tiles;
+        for (int64_t tile = base; tile < end; tile++) {
+            // view owning this tile (uniform across the wave)
+            int32_t lo = 0, hi = n_views - 1;
+            while (lo < hi) {
+                const int32_t mid = (lo + hi + 1) >> 1;
+                if (views[mid].first_tile <= tile) lo = mid;
+                else hi = mid - 1;
+            }
+            v = __builtin_amdgcn_readfirstlane(lo);
+            const ViewDesc& V = views[v];
+            const int32_t local = (int32_t)(tile - V.first_tile);
+            const int32_t tx0 = (local % V.tiles_x) * tw;
+            const int32_t ty0 = (local / V.tiles_x) * th;
+            for (int32_t pass = 0; pass < passes; pass++) {
+                const int32_t px = tx0 + lane % tw;
+                const int32_t row = ty0 + pass * rows_per_pass + lane / tw;
+                if (px < V.target.width && row < V.target.height) {
+                    float vx, vy;
+                    dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
+                    dev::PixelResult p;
+                    dev::shade_pixel(g, V.frame, vx, vy, p);
+                    dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
+                    rays += p.rays;
+                    iters += p.iters;
+                    pixels += 1;
+                }
+            }
+        }
+    }
+    flush_counters(counters, rays, iters, pixels);
+}
+
+__global__ void k_queue_reset(unsigned int* q) {
+    if (threadIdx.x == 0) q[0] = 0u;
+}
+
+}  // namespace
+
+int launch_render_plain(const GridDesc& g, const FrameDesc& f, const vrt_target& t, unsigned long long* counters,
+                        hipStream_t s) {
+    const int64_t tiles = (int64_t)((t.width + 7) / 8) * ((t.height + 7) / 8);
+    const int64_t blocks = (tiles + kPlainWaves - 1) / kPlainWaves;
+    if (blocks <= 0) return VRT_OK;
+    if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "frame too large");
+    hipLaunchKernelGGL(k_render_plain, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t, counters);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const float* dir, const int32_t* dist,
+                     int32_t render_dist, int64_t* hit, float* hit_pos, uint8_t* face, uint32_t* steps,
+                     hipStream_t s) {
+    if (n <= 0) return VRT_OK;
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "too many rays");
+    hipLaunchKernelGGL(k_cast_rays, dim3((uint32_t)blocks), dim3(256), 0, s, g, n, start, dir, dist,
+                       (float)render_dist, hit, hit_pos, face, steps);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu) {
+    (void)cfg;
+    int blocks = 0;
+    VRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_batch, 256, 0));
+    *waves_per_cu = blocks * 4;
+    return VRT_OK;
+}
+
+int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_views, int64_t total_tiles,
+                        const vrt_batch_config& cfg, int32_t n_workgroups, unsigned int* queue,
+                        unsigned long long* counters, hipStream_t s) {
+    if (n_workgroups <= 0) return fail(VRT_ERR_INVALID, "no workgroups");
+    hipLaunchKernelGGL(k_render_batch, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev, n_views,
+                       total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_queue_reset(unsigned int* queue, hipStream_t s) {
+    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(64), 0, s, queue);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+}  // namespace vrt

@@ -1,0 +1,429 @@
+// vrt_server.hip — served form: the world stays resident on the device,
+// removeSphere edits are applied there between frames, and render requests
+// are admitted by framebuffer memory into continuous batches.
+//
+// Ordering.  Requests and edits form one FIFO.  A request sees every edit
+// queued before it and none after: an edit is launched on the edit stream
+// after waiting (hipStreamWaitEvent) for every in-flight batch, and every
+// later batch waits for that edit's event.  The host never blocks on an edit.
+//
+// Continuous batching.  Each pump admits queued requests, oldest first, while
+// their frame buffers fit in the budget and a batch slot (stream) is free, and
+// launches them as one persistent multi-view launch on that slot's stream.
+// Requests arriving while earlier batches run go into the next slot, so their
+// work overlaps the tail of the running batches on the device.  A completed
+// request keeps its buffers (and its share of the budget) until released.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "vrt_internal.h"
+
+namespace vrt {
+int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& cfg, std::vector<ViewDesc>& out,
+                     int64_t& total_tiles);
+int persistent_workgroups(vrt_ctx* c, const vrt_batch_config& cfg, int64_t total_tiles, int32_t* out);
+}  // namespace vrt
+
+namespace {
+
+struct Request {
+    uint64_t id = 0;
+    vrt_frame frame{};
+    int32_t width = 0, height = 0;
+    uint64_t bytes = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    uint64_t world_version = 0;
+    vrt_target target{};
+    void* mem = nullptr;
+    bool done = false;
+};
+
+struct Item {  // FIFO entry: a request or an edit
+    bool is_edit = false;
+    uint64_t id = 0;  // request id
+    int32_t cx = 0, cy = 0, cz = 0, r = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    vrt::ViewDesc* views_dev = nullptr;
+    vrt::ViewDesc* views_host = nullptr;  // pinned staging
+    unsigned int* queue = nullptr;
+    bool busy = false;
+    std::vector<uint64_t> ids;
+};
+
+}  // namespace
+
+struct vrt_server {
+    vrt_ctx* ctx = nullptr;
+    vrt_server_config cfg{};
+    double t0 = 0;
+    uint64_t next_id = 1;
+    uint64_t version_queued = 0;   // edits submitted
+    uint64_t version_applied = 0;  // edits launched on the device
+    hipStream_t edit_stream = nullptr;
+    hipEvent_t edit_done = nullptr;
+    bool have_edit = false;
+    std::deque<Item> fifo;
+    std::map<uint64_t, Request> reqs;
+    std::vector<Slot> slots;
+    std::deque<uint64_t> completed;
+    uint64_t bytes_in_use = 0;
+    int32_t queued = 0;
+    // Released frame buffers kept for reuse, keyed by size: hipFree would
+    // synchronise the device and stall batches running on other slots.
+    std::multimap<uint64_t, void*> pool;
+    uint64_t pool_bytes = 0;
+};
+
+namespace vrt {
+namespace {
+
+uint64_t frame_bytes(const vrt_server_config& c, int32_t w, int32_t h) {
+    const uint64_t px = (uint64_t)w * (uint64_t)h;
+    uint64_t per = 3;                    // rgb
+    if (c.want_depth) per += 4;          // depth
+    if (c.want_aux) per += 8 + 1 + 4 + 4 + 4;  // hit, face, eval, occ, steps
+    return px * per + 6 * 256;           // each buffer 256-byte aligned
+}
+
+uint64_t align256(uint64_t v) { return (v + 255) & ~(uint64_t)255; }
+
+int alloc_target(vrt_server* s, Request& r) {
+    const uint64_t px = (uint64_t)r.width * (uint64_t)r.height;
+    void* mem = nullptr;
+    auto hit = s->pool.find(r.bytes);
+    if (hit != s->pool.end()) {
+        mem = hit->second;
+        s->pool.erase(hit);
+        s->pool_bytes -= r.bytes;
+    } else {
+        // keep cached plus live buffers within the budget
+        while (!s->pool.empty() && s->pool_bytes + s->bytes_in_use + r.bytes > s->cfg.framebuffer_budget) {
+            auto big = std::prev(s->pool.end());
+            hipFree(big->second);
+            s->pool_bytes -= big->first;
+            s->pool.erase(big);
+        }
+        VRT_HIP(hipMalloc(&mem, r.bytes));
+    }
+    uint8_t* p = (uint8_t*)mem;
+    std::memset(&r.target, 0, sizeof(r.target));
+    r.target.width = r.width;
+    r.target.height = r.height;
+    r.target.rgb = p;
+    p += align256(px * 3);
+    if (s->cfg.want_depth) {
+        r.target.depth = (float*)p;
+        p += align256(px * 4);
+    }
+    if (s->cfg.want_aux) {
+        r.target.hit = (int64_t*)p;
+        p += align256(px * 8);
+        r.target.face = p;
+        p += align256(px);
+        r.target.shadow_eval = (uint32_t*)p;
+        p += align256(px * 4);
+        r.target.shadow_occ = (uint32_t*)p;
+        p += align256(px * 4);
+        r.target.steps = (uint32_t*)p;
+    }
+    r.mem = mem;
+    return VRT_OK;
+}
+
+double clock_of(vrt_server* s) { return now_seconds() - s->t0; }
+
+int retire(vrt_server* s) {
+    for (auto& sl : s->slots) {
+        if (!sl.busy) continue;
+        hipError_t e = hipEventQuery(sl.done);
+        if (e == hipErrorNotReady) continue;
+        if (e != hipSuccess) return hip_fail(e, "batch completion");
+        const double t = clock_of(s);
+        for (uint64_t id : sl.ids) {
+            auto it = s->reqs.find(id);
+            if (it == s->reqs.end()) continue;
+            it->second.t_done = t;
+            it->second.done = true;
+            s->completed.push_back(id);
+        }
+        sl.ids.clear();
+        sl.busy = false;
+    }
+    return VRT_OK;
+}
+
+int apply_edit(vrt_server* s, const Item& it) {
+    vrt_ctx* c = s->ctx;
+    VRT_HIP(hipStreamWaitEvent(s->edit_stream, s->edit_done, 0));
+    for (auto& sl : s->slots)
+        if (sl.busy) VRT_HIP(hipStreamWaitEvent(s->edit_stream, sl.done, 0));
+    hipStream_t keep = c->stream;
+    c->stream = s->edit_stream;
+    int rc = vrt_remove_sphere(c, it.cx, it.cy, it.cz, it.r);
+    c->stream = keep;
+    if (rc) return rc;
+    VRT_HIP(hipEventRecord(s->edit_done, s->edit_stream));
+    s->have_edit = true;
+    s->version_applied++;
+    return VRT_OK;
+}
+
+int launch_slot(vrt_server* s, Slot& sl, std::vector<uint64_t>& ids) {
+    vrt_ctx* c = s->ctx;
+    std::vector<vrt_view> views(ids.size());
+    for (size_t i = 0; i < ids.size(); i++) {
+        Request& r = s->reqs[ids[i]];
+        views[i].frame = r.frame;
+        views[i].target = r.target;
+    }
+    vrt_batch_config cfg = s->cfg.batch.tile_w ? s->cfg.batch : c->tuned_cfg;
+    std::vector<ViewDesc> descs;
+    int64_t total = 0;
+    if (int rc = build_view_descs(views.data(), (int32_t)views.size(), cfg, descs, total)) return rc;
+    int32_t wgs = 0;
+    if (int rc = persistent_workgroups(c, cfg, total, &wgs)) return rc;
+    // The staging buffer is reused only after the slot's previous batch
+    // finished (slot not busy), so the async copy never races a rewrite.
+    std::memcpy(sl.views_host, descs.data(), descs.size() * sizeof(ViewDesc));
+    if (s->have_edit) VRT_HIP(hipStreamWaitEvent(sl.stream, s->edit_done, 0));
+    VRT_HIP(hipMemcpyAsync(sl.views_dev, sl.views_host, descs.size() * sizeof(ViewDesc), hipMemcpyHostToDevice,
+                           sl.stream));
+    if (int rc = launch_queue_reset(sl.queue, sl.stream)) return rc;
+    if (int rc = launch_render_batch(c->grid, sl.views_dev, (int32_t)descs.size(), total, cfg, wgs, sl.queue,
+                                     c->counters, sl.stream))
+        return rc;
+    VRT_HIP(hipEventRecord(sl.done, sl.stream));
+    const double t = clock_of(s);
+    for (uint64_t id : ids) {
+        Request& r = s->reqs[id];
+        r.t_start = t;
+        r.world_version = s->version_applied;
+    }
+    sl.ids = ids;
+    sl.busy = true;
+    c->draws += ids.size();
+    return VRT_OK;
+}
+
+// Admit and launch; returns the number of batches launched.
+int admit(vrt_server* s, int* launched) {
+    *launched = 0;
+    for (;;) {
+        while (!s->fifo.empty() && s->fifo.front().is_edit) {
+            if (int rc = apply_edit(s, s->fifo.front())) return rc;
+            s->fifo.pop_front();
+        }
+        if (s->fifo.empty()) return VRT_OK;
+        Slot* free_slot = nullptr;
+        for (auto& sl : s->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        if (!free_slot) return VRT_OK;
+        std::vector<uint64_t> ids;
+        while (!s->fifo.empty() && !s->fifo.front().is_edit && (int32_t)ids.size() < s->cfg.max_batch_views) {
+            Request& r = s->reqs[s->fifo.front().id];
+            if (s->bytes_in_use + r.bytes > s->cfg.framebuffer_budget) break;
+            if (int rc = alloc_target(s, r)) {
+                if (ids.empty()) return rc;
+                break;  // device memory short: launch what we have
+            }
+            s->bytes_in_use += r.bytes;
+            ids.push_back(r.id);
+            s->fifo.pop_front();
+            s->queued--;
+        }
+        if (ids.empty()) return VRT_OK;  // budget full: wait for releases
+        if (int rc = launch_slot(s, *free_slot, ids)) return rc;
+        (*launched)++;
+    }
+}
+
+void destroy_server(vrt_server* s) {
+    if (!s) return;
+    if (s->ctx) hipSetDevice(s->ctx->device);
+    for (auto& sl : s->slots) {
+        if (sl.stream) hipStreamSynchronize(sl.stream);
+        if (sl.done) hipEventDestroy(sl.done);
+        if (sl.views_dev) hipFree(sl.views_dev);
+        if (sl.views_host) hipHostFree(sl.views_host);
+        if (sl.queue) hipFree(sl.queue);
+        if (sl.stream) hipStreamDestroy(sl.stream);
+    }
+    if (s->edit_stream) hipStreamSynchronize(s->edit_stream);
+    for (auto& kv : s->reqs)
+        if (kv.second.mem) hipFree(kv.second.mem);
+    for (auto& kv : s->pool) hipFree(kv.second);
+    if (s->edit_done) hipEventDestroy(s->edit_done);
+    if (s->edit_stream) hipStreamDestroy(s->edit_stream);
+    delete s;
+}
+
+}  // namespace
+}  // namespace vrt
+
+using namespace vrt;
+
+extern "C" {
+
+int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** out) {
+    if (!ctx || !ctx->grid.vox || !cfg || !out) return fail(VRT_ERR_INVALID, "invalid arguments");
+    *out = nullptr;
+    if (cfg->max_batch_views < 1 || cfg->max_batch_views > 4096 || cfg->streams < 1 || cfg->streams > 16 ||
+        cfg->framebuffer_budget == 0)
+        return fail(VRT_ERR_INVALID, "server config out of range");
+    VRT_HIP(hipSetDevice(ctx->device));
+    vrt_server* s = new vrt_server();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    s->t0 = now_seconds();
+    auto bail = [&](int rc) {
+        destroy_server(s);
+        return rc;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&s->edit_stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamCreate"));
+    if ((e = hipEventCreateWithFlags(&s->edit_done, hipEventDisableTiming)) != hipSuccess)
+        return bail(hip_fail(e, "hipEventCreate"));
+    // edits and batches must also follow whatever the context queued before
+    if ((e = hipEventRecord(s->edit_done, ctx->stream)) != hipSuccess) return bail(hip_fail(e, "hipEventRecord"));
+    s->have_edit = true;
+    s->slots.resize(cfg->streams);
+    const size_t vbytes = (size_t)cfg->max_batch_views * sizeof(ViewDesc);
+    for (auto& sl : s->slots) {
+        if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
+            return bail(hip_fail(e, "hipStreamCreate"));
+        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
+            return bail(hip_fail(e, "hipEventCreate"));
+        if ((e = hipMalloc(&sl.views_dev, vbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc"));
+        if ((e = hipHostMalloc(&sl.views_host, vbytes, hipHostMallocDefault)) != hipSuccess)
+            return bail(hip_fail(e, "hipHostMalloc"));
+        if ((e = hipMalloc(&sl.queue, 64)) != hipSuccess) return bail(hip_fail(e, "hipMalloc"));
+    }
+    *out = s;
+    return VRT_OK;
+}
+
+int vrt_server_destroy(vrt_server* s) {
+    if (!s) return fail(VRT_ERR_INVALID, "invalid server");
+    destroy_server(s);
+    return VRT_OK;
+}
+
+double vrt_server_clock(vrt_server* s) { return s ? clock_of(s) : 0.0; }
+
+int vrt_server_submit(vrt_server* s, const vrt_frame* f, int32_t width, int32_t height, double t_submit,
+                      uint64_t* id) {
+    if (!s || !f) return fail(VRT_ERR_INVALID, "invalid arguments");
+    if (int rc = check_frame(*f)) return rc;
+    if (width <= 0 || height <= 0 || (int64_t)width * height > ((int64_t)1 << 31))
+        return fail(VRT_ERR_INVALID, "invalid frame size");
+    Request r;
+    r.id = s->next_id++;
+    r.frame = *f;
+    r.width = width;
+    r.height = height;
+    r.bytes = frame_bytes(s->cfg, width, height);
+    if (r.bytes > s->cfg.framebuffer_budget) return fail(VRT_ERR_BUDGET, "request exceeds the framebuffer budget");
+    r.t_submit = t_submit;
+    s->reqs[r.id] = r;
+    Item it;
+    it.id = r.id;
+    s->fifo.push_back(it);
+    s->queued++;
+    if (id) *id = r.id;
+    return VRT_OK;
+}
+
+int vrt_server_submit_edit(vrt_server* s, int32_t cx, int32_t cy, int32_t cz, int32_t radius, uint64_t* version) {
+    if (!s) return fail(VRT_ERR_INVALID, "invalid server");
+    if (radius < -3 || radius > 4096) return fail(VRT_ERR_INVALID, "radius must be in [-3, 4096]");
+    Item it;
+    it.is_edit = true;
+    it.cx = cx;
+    it.cy = cy;
+    it.cz = cz;
+    it.r = radius;
+    s->fifo.push_back(it);
+    s->version_queued++;
+    if (version) *version = s->version_queued;
+    return VRT_OK;
+}
+
+int vrt_server_pump(vrt_server* s, int32_t wait) {
+    if (!s) return fail(VRT_ERR_INVALID, "invalid server");
+    VRT_HIP(hipSetDevice(s->ctx->device));
+    if (int rc = retire(s)) return rc;
+    int launched = 0;
+    if (int rc = admit(s, &launched)) return rc;
+    if (wait && launched == 0 && s->completed.empty()) {
+        // nothing moved: block on the oldest in-flight batch
+        Slot* oldest = nullptr;
+        for (auto& sl : s->slots)
+            if (sl.busy && (!oldest || s->reqs[sl.ids.front()].t_start < s->reqs[oldest->ids.front()].t_start))
+                oldest = &sl;
+        if (oldest) {
+            VRT_HIP(hipEventSynchronize(oldest->done));
+            if (int rc = retire(s)) return rc;
+            if (int rc = admit(s, &launched)) return rc;
+        }
+    }
+    return VRT_OK;
+}
+
+int vrt_server_poll(vrt_server* s, vrt_result* out, int32_t max, int32_t* n) {
+    if (!s || (!out && max > 0)) return fail(VRT_ERR_INVALID, "invalid arguments");
+    int32_t k = 0;
+    while (k < max && !s->completed.empty()) {
+        const uint64_t id = s->completed.front();
+        s->completed.pop_front();
+        const Request& r = s->reqs[id];
+        vrt_result& o = out[k++];
+        o.id = r.id;
+        o.world_version = r.world_version;
+        o.width = r.width;
+        o.height = r.height;
+        o.t_submit = r.t_submit;
+        o.t_start = r.t_start;
+        o.t_done = r.t_done;
+        o.target = r.target;
+    }
+    if (n) *n = k;
+    return VRT_OK;
+}
+
+int vrt_server_release(vrt_server* s, uint64_t id) {
+    if (!s) return fail(VRT_ERR_INVALID, "invalid server");
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end() || it->second.mem == nullptr || !it->second.done)
+        return fail(VRT_ERR_NOTFOUND, "no completed request with this id");
+    s->pool.emplace(it->second.bytes, it->second.mem);
+    s->pool_bytes += it->second.bytes;
+    s->bytes_in_use -= it->second.bytes;
+    s->reqs.erase(it);
+    return VRT_OK;
+}
+
+int vrt_server_status(vrt_server* s, int32_t* queued, int32_t* inflight, int32_t* done, uint64_t* bytes_in_use) {
+    if (!s) return fail(VRT_ERR_INVALID, "invalid server");
+    int32_t fl = 0;
+    for (auto& sl : s->slots) fl += sl.busy ? (int32_t)sl.ids.size() : 0;
+    if (queued) *queued = s->queued;
+    if (inflight) *inflight = fl;
+    if (done) *done = (int32_t)s->completed.size();
+    if (bytes_in_use) *bytes_in_use = s->bytes_in_use;
+    return VRT_OK;
+}
+
+}  // extern "C"
