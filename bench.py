@@ -39,6 +39,12 @@ def load():
     return vrt
 
 
+def grid_ptr(vrt, ctx):
+    p = vrt.P()
+    vrt.call("vrt_grid_device_ptr", ctx.h, vrt.C.byref(p))
+    return p.value
+
+
 def plain_bench(vrt, steps, warmup):
     sc = vrt.make_scene("c2", seed=1)
     ctx = vrt.Context(sc.dims)
@@ -47,6 +53,18 @@ def plain_bench(vrt, steps, warmup):
     ctx.build_depth_field()
     ctx.sync()
     log("c2 world %s, depth field %.3f s" % (sc.dims, time.time() - t0))
+    # One device-to-device copy of the whole grid (256 MiB read + 256 MiB
+    # written by k_copy): a plain copy of known size that calibrates the
+    # profiler's FETCH_SIZE / WRITE_SIZE counters for this run.
+    snapshot = vrt.DeviceBuffer(4 * sc.world.size)
+    vrt.hip_check(vrt.hip().hipMemcpy(snapshot.ptr, vrt.C.c_void_p(grid_ptr(vrt, ctx)), snapshot.nbytes, vrt.D2D),
+                  "hipMemcpy D2D")
+    # a device-to-device hipMemcpy may return before the copy is done, and the
+    # context's stream does not wait for the null stream
+    vrt.hip_check(vrt.hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+    vrt.call("vrt_upload_device", ctx.h, snapshot.addr)
+    ctx.sync()
+    snapshot.free()
     tgt = vrt.Target(sc.width, sc.height, depth=True, aux=False)
     ts = tgt.struct()
     stream = ctx.stream

@@ -11,13 +11,21 @@ pytestmark = pytest.mark.gpu
 RGB_TOL = 1  # LSB per channel
 
 
+def same_floats(a, b):
+    """Bitwise equal, or NaN on both sides: IEEE 754 leaves the sign and payload
+    of a NaN made by an invalid operation open (x86 gives 0xFFC00000, gfx950
+    0x7FC00000), and no comparison or int() conversion depends on them."""
+    a, b = np.asarray(a, np.float32), np.asarray(b, np.float32)
+    return bool(np.all((a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))))
+
+
 def assert_frames_match(g, r, label=""):
     for k in ("hit", "face", "shadow_eval", "shadow_occ", "steps"):
         bad = np.argwhere(g[k] != r[k])
         assert bad.size == 0, "%s %s differs at %d pixels, first %s: gpu %s oracle %s" % (
             label, k, len(bad), bad[0].tolist(), g[k][tuple(bad[0])], r[k][tuple(bad[0])])
     if "depth" in g:
-        assert np.array_equal(g["depth"].view(np.uint32), r["depth"].view(np.uint32)), label + " depth differs"
+        assert same_floats(g["depth"], r["depth"]), label + " depth differs"
     d = np.abs(g["rgb"].astype(np.int16) - r["rgb"].astype(np.int16))
     assert int(d.max()) <= RGB_TOL, "%s rgb differs by %d" % (label, int(d.max()))
     assert g["rays"] == r["rays"], "%s rays %d vs %d" % (label, g["rays"], r["rays"])
@@ -86,7 +94,7 @@ def test_cast_rays_match_oracle(gpu, orc):
     for k in ("hit", "face", "steps"):
         bad = np.nonzero(g[k] != r[k])[0]
         assert bad.size == 0, "%s differs for %d rays, first %d" % (k, bad.size, bad[0])
-    assert np.array_equal(g["hit_pos"].view(np.uint32), r["hit_pos"].view(np.uint32))
+    assert same_floats(g["hit_pos"], r["hit_pos"])
     ctx.close()
 
 
@@ -158,3 +166,49 @@ def test_c2_frame_matches_oracle(gpu, orc):
     r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height)
     assert_frames_match(g, r, "c2")
     ctx.close()
+
+
+def test_cast_rays_extreme_values_match_oracle(gpu, orc):
+    """int(float) at NaN, +-inf and beyond int32 range, zero and denormal
+    direction components, huge budgets: the conversions saturate and NaN goes
+    to 0 on both sides."""
+    sc = gpu.make_scene("tiny", seed=8)
+    wdf = orc.depth_field(sc.world, sc.dims)
+    nan, inf = float("nan"), float("inf")
+    starts = [[nan, 40.0, 40.0], [40.0, nan, 40.0], [inf, 40.0, 40.0], [-inf, 40.0, 40.0], [3e9, 40.0, 40.0],
+              [-3e9, 40.0, 40.0], [2147483520.0, 40.0, 40.0], [40.5, 45.5, 40.5], [40.5, 45.5, 40.5],
+              [40.5, 45.5, 40.5], [40.5, 45.5, 40.5], [1e-40, 50.0, 1e-40], [95.99999, 63.99999, 95.99999]]
+    dirs = [[1.0, 0.0, 0.0], [0.0, -1.0, 0.0], [-1.0, 0.0, 0.0], [1.0, 0.0, 0.0], [-1.0, 0.0, 0.0],
+            [1.0, 0.0, 0.0], [-1.0, 0.0, 0.0], [nan, 0.5, 0.5], [inf, -1.0, 0.0], [1e-45, -1.0, 1e-45],
+            [0.0, 0.0, 0.0], [0.7071, -0.7071, 0.0], [-0.57735, -0.57735, -0.57735]]
+    n = len(starts)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(wdf)
+    for budget in (384, 1 << 24):
+        dist = np.full(n, budget, np.int32)
+        g = ctx.cast_rays(starts, dirs, dist, budget)
+        r = orc.cast_rays(wdf, sc.dims, starts, dirs, dist, budget)
+        for k in ("hit", "face", "steps"):
+            assert np.array_equal(g[k], r[k]), (budget, k, g[k], r[k])
+        assert same_floats(g["hit_pos"], r["hit_pos"])
+    ctx.close()
+
+
+def test_device_copy_and_generator(gpu):
+    """vrt_upload_device copies a whole grid on the device; the device world
+    generator writes the same words as the host generator."""
+    for name, seed in (("tiny", 3), ("c1", 2)):
+        cfg = gpu.CONFIGS[name]
+        hmin, hmax = cfg.get("heights", (16, 96))
+        p = gpu.world_params(cfg["dims"], seed, caves=cfg["caves"], boxes=cfg["boxes"] + 50, height_min=hmin,
+                             height_max=hmax)
+        host = gpu.generate_world(p)
+        ctx = gpu.Context(cfg["dims"])
+        ctx.generate(p)
+        dev = ctx.download()
+        assert np.array_equal(dev, host), "%s: device generator differs at %d voxels" % (name, int((dev != host).sum()))
+        other = gpu.DeviceBuffer(host.nbytes)
+        other.upload(host[::-1].copy())
+        gpu.call("vrt_upload_device", ctx.h, other.addr)
+        assert np.array_equal(ctx.download(), host[::-1])
+        ctx.close()

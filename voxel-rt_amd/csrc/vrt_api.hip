@@ -289,9 +289,7 @@ int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels) {
 int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels) {
     if (int rc = check_ctx(ctx)) return rc;
     if (!device_voxels) return fail(VRT_ERR_INVALID, "null voxels");
-    VRT_HIP(hipMemcpyAsync(ctx->grid.vox, device_voxels, ctx->grid.count * sizeof(int32_t),
-                           hipMemcpyDeviceToDevice, ctx->stream));
-    return VRT_OK;
+    return launch_copy(ctx->grid.vox, device_voxels, ctx->grid.count, ctx->stream);
 }
 
 // updatePartialGeometry, src/render.cpp:204-223: corners swapped when the

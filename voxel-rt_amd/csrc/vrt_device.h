@@ -20,11 +20,12 @@ namespace vrt {
 namespace dev {
 
 // GLSL int(float): truncation toward zero; out-of-range saturates, NaN -> 0.
+// That is exactly what v_cvt_i32_f32 does, so it is issued directly (C++'s
+// (int) of an out-of-range float is undefined and the guards would branch).
 __device__ __forceinline__ int32_t f2i(float v) {
-    if (v != v) return 0;
-    if (v >= 2147483648.0f) return INT32_MAX;
-    if (v <= -2147483648.0f) return INT32_MIN;
-    return (int32_t)v;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
 }
 
 // int(sign(v)) (src/fshader.glsl:71); sign(NaN) = 0.
@@ -55,12 +56,38 @@ struct Hit {
     uint32_t iters;  // loop iterations (stepCount increments)
 };
 
+// Voxel index arithmetic: 32-bit with 24-bit multiplies when every index and
+// partial product fits (X*Y*Z <= 2^32, X and X*Y < 2^24), else 64-bit.
+struct Idx32 {
+    using T = uint32_t;
+    __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
+        return (uint32_t)x + __umul24((uint32_t)y, g.X) + __umul24((uint32_t)z, (uint32_t)g.XY);
+    }
+    __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
+        return axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.X : (uint32_t)sz * (uint32_t)g.XY);
+    }
+};
+struct Idx64 {
+    using T = uint64_t;
+    __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
+        return (uint64_t)(int64_t)x + (uint64_t)g.X * (uint64_t)(int64_t)y + g.XY * (uint64_t)(int64_t)z;
+    }
+    __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
+        return axis == 0 ? (uint64_t)(int64_t)sx
+                         : (axis == 1 ? (uint64_t)(int64_t)sy * g.X : (uint64_t)(int64_t)sz * g.XY);
+    }
+};
+
 // castRay, src/fshader.glsl:59-129.  `lim` = min(float(dist), float(RENDER_DIST)):
 // the loop test `t < dist && t < RENDER_DIST` of :83 is `t < lim` for any t.
-// The voxel index of the current cell is kept incrementally and is only used
-// while the cell is inside the grid (per-axis bounds, :42-45).
+// The voxel index of the current cell is kept incrementally (wrapping) and
+// is only used while the cell is inside the grid (per-axis bounds, :42-45):
+// between two in-grid cells every step was checked, so it never wrapped.
+// The axis choice is computed with selects so the wave does not branch.
+template <typename IX>
 __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, float sz, float dx, float dy,
                                          float dz, float lim, Hit& h) {
+    using T = typename IX::T;
     int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
     const int32_t stx = isign(dx), sty = isign(dy), stz = isign(dz);
     const int32_t fx = stx > 0, fy = sty > 0, fz = stz > 0;
@@ -70,14 +97,11 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     float ix = ((float)(cx + fx) - sx) / dx;
     float iy = ((float)(cy + fy) - sy) / dy;
     float iz = ((float)(cz + fz) - sz) / dz;
-
-    // Index deltas of one step on each axis (wrapping; only read in-grid).
-    const uint64_t X = g.X;
-    const uint64_t XY = g.XY;
-    const uint64_t sdx = (uint64_t)(int64_t)stx;
-    const uint64_t sdy = (uint64_t)(int64_t)sty * X;
-    const uint64_t sdz = (uint64_t)(int64_t)stz * XY;
-    uint64_t idx = (uint64_t)(int64_t)cx + X * (uint64_t)(int64_t)cy + XY * (uint64_t)(int64_t)cz;
+    const T sdx = IX::delta(g, stx, sty, stz, 0);
+    const T sdy = IX::delta(g, stx, sty, stz, 1);
+    const T sdz = IX::delta(g, stx, sty, stz, 2);
+    T idx = IX::make(g, cx, cy, cz);
+    const int32_t* __restrict__ vox = g.vox;
 
     float cur = 0.0f;
     float trav = 0.0f;
@@ -89,28 +113,19 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
         trav += 1.0f;
         const bool bx = (ix < iy) & (ix < iz);
         const bool by = (iy < ix) & (iy < iz);
-        if (bx) {
-            cur = ix;
-            cx += stx;
-            ix += ddx;
-            idx += sdx;
-            axis = 0;
-        } else if (by) {
-            cur = iy;
-            cy += sty;
-            iy += ddy;
-            idx += sdy;
-            axis = 1;
-        } else {
-            cur = iz;
-            cz += stz;
-            iz += ddz;
-            idx += sdz;
-            axis = 2;
-        }
+        const bool bz = !(bx | by);
+        cur = bx ? ix : (by ? iy : iz);
+        cx = bx ? cx + stx : cx;
+        cy = by ? cy + sty : cy;
+        cz = bz ? cz + stz : cz;
+        ix = bx ? ix + ddx : ix;
+        iy = by ? iy + ddy : iy;
+        iz = bz ? iz + ddz : iz;
+        idx += bx ? sdx : (by ? sdy : sdz);
+        axis = bx ? 0 : (by ? 1 : 2);
         const bool inb = ((uint32_t)cx < g.X) & ((uint32_t)cy < g.Y) & ((uint32_t)cz < g.Z);
         if (!inb) break;
-        const int32_t v = g.vox[idx];
+        const int32_t v = vox[idx];
         if (v >= 0) {
             h.pos[0] = dx * cur + sx;
             h.pos[1] = dy * cur + sy;
@@ -131,7 +146,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             ix = ((float)(cx + fx) - sx) / dx;
             iy = ((float)(cy + fy) - sy) / dy;
             iz = ((float)(cz + fz) - sz) / dz;
-            idx = (uint64_t)(int64_t)cx + X * (uint64_t)(int64_t)cy + XY * (uint64_t)(int64_t)cz;
+            idx = IX::make(g, cx, cy, cz);
         }
     }
     h.index = result;
@@ -153,6 +168,7 @@ struct PixelResult {
 };
 
 // main(), src/fshader.glsl:131-190, for the fragment at vPos = (vx, vy).
+template <typename IX>
 __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& F, float vx, float vy,
                                             PixelResult& o) {
     const vrt_frame& f = F.f;
@@ -169,7 +185,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
     const float d2 = (M[2] * rx + M[6] * ry) + (M[10] * rz + M[14] * 0.0f);
 
     Hit h;
-    cast_ray(g, f.cam_pos[0], f.cam_pos[1], f.cam_pos[2], d0, d1, d2, F.render_lim, h);
+    cast_ray<IX>(g, f.cam_pos[0], f.cam_pos[1], f.cam_pos[2], d0, d1, d2, F.render_lim, h);
     uint32_t iters = h.iters;
     uint32_t rays = 1;
     uint32_t eval = 0, occ = 0;
@@ -206,7 +222,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
         // sun shadow (:152-156)
         {
             Hit s;
-            cast_ray(g, hx + tlx * 0.001f, hy + tly * 0.001f, hz + tlz * 0.001f, tlx, tly, tlz, F.render_lim, s);
+            cast_ray<IX>(g, hx + tlx * 0.001f, hy + tly * 0.001f, hz + tlz * 0.001f, tlx, tly, tlz, F.render_lim, s);
             iters += s.iters;
             rays++;
             eval |= 1u;
@@ -238,7 +254,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             float lim = (float)f2i(dist + 1.0f);
             lim = lim < F.render_lim ? lim : F.render_lim;
             Hit s;
-            cast_ray(g, hx + ux1 * 0.001f, hy + uy1 * 0.001f, hz + uz1 * 0.001f, ux1, uy1, uz1, lim, s);
+            cast_ray<IX>(g, hx + ux1 * 0.001f, hy + uy1 * 0.001f, hz + uz1 * 0.001f, ux1, uy1, uz1, lim, s);
             iters += s.iters;
             rays++;
             eval |= 2u << i;

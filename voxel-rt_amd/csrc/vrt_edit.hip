@@ -112,6 +112,16 @@ __global__ void k_set_voxel(int32_t* vox, uint64_t i, int32_t v) {
     if (threadIdx.x == 0) vox[i] = v;
 }
 
+// Grid copy, 16 bytes per lane per step (a plain streaming copy of known size).
+__global__ void __launch_bounds__(256) k_copy(int4* __restrict__ dst, const int4* __restrict__ src, uint64_t n16,
+                                               int32_t* __restrict__ dst_tail, const int32_t* __restrict__ src_tail,
+                                               uint32_t tail) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint64_t i = t; i < n16; i += stride) dst[i] = src[i];
+    if (t < tail) dst_tail[t] = src_tail[t];
+}
+
 __global__ void __launch_bounds__(256) k_fill(int32_t* p, uint64_t n, int32_t v) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -175,6 +185,20 @@ int launch_carve_sphere(const GridDesc& g, int32_t cx, int32_t cy, int32_t cz, i
 int launch_set_voxel(const GridDesc& g, int64_t index, int32_t v, hipStream_t s) {
     if (index < 0 || (uint64_t)index >= g.count) return fail(VRT_ERR_INVALID, "voxel index out of range");
     hipLaunchKernelGGL(k_set_voxel, dim3(1), dim3(64), 0, s, g.vox, (uint64_t)index, v);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_copy(int32_t* dst, const int32_t* src, uint64_t n, hipStream_t s) {
+    if (n == 0) return VRT_OK;
+    if (((uintptr_t)dst | (uintptr_t)src) & 15) return fail(VRT_ERR_INVALID, "copy buffers must be 16-byte aligned");
+    const uint64_t n16 = n / 4;
+    const uint32_t tail = (uint32_t)(n % 4);
+    uint64_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_copy, dim3((uint32_t)blocks), dim3(256), 0, s, (int4*)dst, (const int4*)src, n16,
+                       dst + 4 * n16, src + 4 * n16, tail);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }

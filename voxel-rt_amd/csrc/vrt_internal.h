@@ -27,6 +27,8 @@ struct GridDesc {
     uint32_t X, Y, Z;
     uint64_t XY;
     uint64_t count;
+    // Index fits the 32-bit / 24-bit-multiply form (see Idx32).
+    bool small() const { return count <= 0xffffffffull && X < (1u << 24) && XY < (1ull << 24); }
 };
 
 // One frame as the kernels see it: vrt_frame plus derived per-frame values.
@@ -71,6 +73,7 @@ int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, 
 int launch_carve_sphere(const GridDesc& g, int32_t cx, int32_t cy, int32_t cz, int32_t r, hipStream_t s);
 int launch_set_voxel(const GridDesc& g, int64_t index, int32_t v, hipStream_t s);
 int launch_fill(int32_t* p, uint64_t n, int32_t v, hipStream_t s);
+int launch_copy(int32_t* dst, const int32_t* src, uint64_t n, hipStream_t s);
 int launch_world_generate(const GridDesc& g, const vrt_world_params& p, const int32_t* cave_list,
                           int32_t n_caves, const int32_t* box_list, int32_t n_boxes,
                           const int32_t* cave_bins, const int32_t* box_bins, int32_t bin_size,

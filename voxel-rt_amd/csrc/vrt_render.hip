@@ -46,6 +46,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
     }
 }
 
+template <typename IX>
 __global__ void __launch_bounds__(kPlainThreads) k_render_plain(GridDesc g, FrameDesc F, vrt_target t,
                                                                 unsigned long long* counters) {
     const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -61,7 +62,7 @@ __global__ void __launch_bounds__(kPlainThreads) k_render_plain(GridDesc g, Fram
             float vx, vy;
             dev::pixel_vpos(t.width, t.height, px, row, vx, vy);
             dev::PixelResult p;
-            dev::shade_pixel(g, F, vx, vy, p);
+            dev::shade_pixel<IX>(g, F, vx, vy, p);
             dev::store_pixel(t, (int64_t)row * t.width + px, p);
             rays = p.rays;
             iters = p.iters;
@@ -71,6 +72,7 @@ __global__ void __launch_bounds__(kPlainThreads) k_render_plain(GridDesc g, Fram
     flush_counters(counters, rays, iters, pixels);
 }
 
+template <typename IX>
 __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const float* __restrict__ start,
                                                    const float* __restrict__ dir,
                                                    const int32_t* __restrict__ dist, float render_lim,
@@ -80,7 +82,7 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
     float lim = (float)dist[i];
     lim = lim < render_lim ? lim : render_lim;
     dev::Hit h;
-    dev::cast_ray(g, start[3 * i], start[3 * i + 1], start[3 * i + 2], dir[3 * i], dir[3 * i + 1], dir[3 * i + 2],
+    dev::cast_ray<IX>(g, start[3 * i], start[3 * i + 1], start[3 * i + 2], dir[3 * i], dir[3 * i + 1], dir[3 * i + 2],
                   lim, h);
     const bool hitp = h.index != -1;
     if (hit) hit[i] = h.index;
@@ -96,6 +98,7 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
 // Persistent multi-view kernel.  Tile shape tw x th with tw*th a multiple of
 // 64: a wave shades 64 pixels per pass (lane -> (lane % tw, lane / tw)) and
 // tw*th/64 passes per tile.  The queue counts tiles; a pop takes `chunk`.
+template <typename IX>
 __global__ void __launch_bounds__(256) k_render_batch(GridDesc g, const ViewDesc* __restrict__ views,
                                                       int32_t n_views, int64_t total_tiles, int32_t tw,
                                                       int32_t th, int32_t chunk, unsigned int* queue,
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(256) k_render_batch(GridDesc g, const ViewDesc
                     float vx, vy;
                     dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
                     dev::PixelResult p;
-                    dev::shade_pixel(g, V.frame, vx, vy, p);
+                    dev::shade_pixel<IX>(g, V.frame, vx, vy, p);
                     dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
                     rays += p.rays;
                     iters += p.iters;
@@ -156,7 +159,12 @@ int launch_render_plain(const GridDesc& g, const FrameDesc& f, const vrt_target&
     const int64_t blocks = (tiles + kPlainWaves - 1) / kPlainWaves;
     if (blocks <= 0) return VRT_OK;
     if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "frame too large");
-    hipLaunchKernelGGL(k_render_plain, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t, counters);
+    if (g.small())
+        hipLaunchKernelGGL(k_render_plain<dev::Idx32>, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t,
+                           counters);
+    else
+        hipLaunchKernelGGL(k_render_plain<dev::Idx64>, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t,
+                           counters);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
@@ -167,8 +175,12 @@ int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const flo
     if (n <= 0) return VRT_OK;
     const int64_t blocks = (n + 255) / 256;
     if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "too many rays");
-    hipLaunchKernelGGL(k_cast_rays, dim3((uint32_t)blocks), dim3(256), 0, s, g, n, start, dir, dist,
-                       (float)render_dist, hit, hit_pos, face, steps);
+    if (g.small())
+        hipLaunchKernelGGL(k_cast_rays<dev::Idx32>, dim3((uint32_t)blocks), dim3(256), 0, s, g, n, start, dir, dist,
+                           (float)render_dist, hit, hit_pos, face, steps);
+    else
+        hipLaunchKernelGGL(k_cast_rays<dev::Idx64>, dim3((uint32_t)blocks), dim3(256), 0, s, g, n, start, dir, dist,
+                           (float)render_dist, hit, hit_pos, face, steps);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
@@ -176,7 +188,7 @@ int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const flo
 int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu) {
     (void)cfg;
     int blocks = 0;
-    VRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_batch, 256, 0));
+    VRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_batch<dev::Idx64>, 256, 0));
     *waves_per_cu = blocks * 4;
     return VRT_OK;
 }
@@ -185,8 +197,12 @@ int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_
                         const vrt_batch_config& cfg, int32_t n_workgroups, unsigned int* queue,
                         unsigned long long* counters, hipStream_t s) {
     if (n_workgroups <= 0) return fail(VRT_ERR_INVALID, "no workgroups");
-    hipLaunchKernelGGL(k_render_batch, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev, n_views,
-                       total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
+    if (g.small())
+        hipLaunchKernelGGL(k_render_batch<dev::Idx32>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,
+                           n_views, total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
+    else
+        hipLaunchKernelGGL(k_render_batch<dev::Idx64>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,
+                           n_views, total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
