@@ -283,6 +283,15 @@ int vrt_server_status(vrt_server* s, int32_t* queued, int32_t* inflight, int32_t
                       uint64_t* bytes_in_use);
 double vrt_server_clock(vrt_server* s);
 
+/* ---- diagnostics (no reference counterpart) ----------------------------- */
+
+/* The DDA divides by a ray's direction component through a reciprocal and
+ * one FMA correction instead of a full IEEE division.  This checks that
+ * path against IEEE division on this device for a = 1 + i/2^23 (every i)
+ * and d = 1 + j/2^23, j in [d_begin, d_begin + d_count): all significand
+ * pairs when run over j in [0, 2^23).  Returns the mismatch count. */
+int vrt_selftest_division(vrt_ctx* ctx, uint32_t d_begin, uint32_t d_count, uint64_t* mismatches);
+
 /* ---- synthetic worlds (benchmark inputs; no reference counterpart) ------ */
 
 typedef struct vrt_world_params {

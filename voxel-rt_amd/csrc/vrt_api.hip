@@ -225,13 +225,14 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
     c->grid.Z = (uint32_t)Z;
     c->grid.XY = (uint64_t)X * (uint64_t)Y;
     c->grid.count = count;
-    if ((e = hipMalloc(&c->counters, 4 * sizeof(unsigned long long))) != hipSuccess)
+    const size_t counter_bytes = (size_t)kCounterSlots * kCounterStride * sizeof(unsigned long long);
+    if ((e = hipMalloc(&c->counters, counter_bytes)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc counters"));
     const uint64_t nbricks = (uint64_t)((X + 7) / 8) * ((Y + 7) / 8) * ((Z + 7) / 8);
     if ((e = hipMalloc(&c->bricks, nbricks)) != hipSuccess) return bail(hip_fail(e, "hipMalloc bricks"));
     int rc = upload_table(c);
     if (rc) return bail(rc);
-    if ((e = hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned long long), c->stream)) != hipSuccess)
+    if ((e = hipMemsetAsync(c->counters, 0, counter_bytes, c->stream)) != hipSuccess)
         return bail(hip_fail(e, "hipMemsetAsync"));
     rc = launch_fill(c->grid.vox, count, -1, c->stream);
     if (rc) return bail(rc);
@@ -502,17 +503,21 @@ int vrt_draw(vrt_ctx* ctx, const vrt_target* t) {
 
 int vrt_read_stats(vrt_ctx* ctx, vrt_stats* out, int32_t reset) {
     if (int rc = check_ctx(ctx)) return rc;
-    unsigned long long h[4] = {0, 0, 0, 0};
-    VRT_HIP(hipMemcpyAsync(h, ctx->counters, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> h((size_t)kCounterSlots * kCounterStride);
+    const size_t bytes = h.size() * sizeof(unsigned long long);
+    VRT_HIP(hipMemcpyAsync(h.data(), ctx->counters, bytes, hipMemcpyDeviceToHost, ctx->stream));
     VRT_HIP(hipStreamSynchronize(ctx->stream));
     if (out) {
-        out->rays = h[0];
-        out->dda_steps = h[1];
-        out->pixels = h[2];
+        out->rays = out->dda_steps = out->pixels = 0;
+        for (int i = 0; i < kCounterSlots; i++) {
+            out->rays += h[(size_t)i * kCounterStride + 0];
+            out->dda_steps += h[(size_t)i * kCounterStride + 1];
+            out->pixels += h[(size_t)i * kCounterStride + 2];
+        }
         out->draws = ctx->draws;
     }
     if (reset) {
-        VRT_HIP(hipMemsetAsync(ctx->counters, 0, sizeof(h), ctx->stream));
+        VRT_HIP(hipMemsetAsync(ctx->counters, 0, bytes, ctx->stream));
         VRT_HIP(hipStreamSynchronize(ctx->stream));
         ctx->draws = 0;
     }
@@ -526,6 +531,25 @@ int vrt_cast_rays(vrt_ctx* ctx, int64_t n, const float* start, const float* dir,
     if (n < 0 || (n > 0 && (!start || !dir || !dist))) return fail(VRT_ERR_INVALID, "invalid ray arrays");
     if (render_dist < 0 || render_dist > (1 << 24)) return fail(VRT_ERR_INVALID, "render_dist must be in [0, 2^24]");
     return launch_cast_rays(ctx->grid, n, start, dir, dist, render_dist, hit, hit_pos, face, steps, ctx->stream);
+}
+
+int vrt_selftest_division(vrt_ctx* ctx, uint32_t d_begin, uint32_t d_count, uint64_t* mismatches) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!mismatches || d_begin >= (1u << 23)) return fail(VRT_ERR_INVALID, "invalid selftest range");
+    unsigned long long* bad = nullptr;
+    VRT_HIP(hipMalloc(&bad, sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned long long), ctx->stream);
+    int rc = e == hipSuccess ? selftest_division(d_begin, d_count, bad, ctx->stream) : hip_fail(e, "hipMemsetAsync");
+    unsigned long long h = 0;
+    if (!rc) {
+        e = hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "selftest copy");
+    }
+    hipFree(bad);
+    if (rc) return rc;
+    *mismatches = h;
+    return VRT_OK;
 }
 
 int vrt_world_generate_device(vrt_ctx* ctx, const vrt_world_params* p) {

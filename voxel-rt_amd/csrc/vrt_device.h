@@ -33,6 +33,28 @@ __device__ __forceinline__ int32_t isign(float v) { return v > 0.0f ? 1 : (v < 0
 
 __device__ __forceinline__ float max0(float v) { return v > 0.0f ? v : 0.0f; }
 
+// a / d rounded to nearest, given y = RN(1/d) (one IEEE division per ray) and
+// dok = (2^-60 <= |d| <= 2^60).  Markstein's correction: q0 = RN(a*y),
+// r = a - q0*d (exact, by FMA), q = RN(q0 + r*y) = RN(a/d).  With a and d
+// in [2^-60, 2^60] every intermediate stays in the normal range, so each step
+// is the same computation on the significands scaled by a power of two; the
+// library's selftest checks all 2^23 x 2^23 significand pairs against IEEE
+// division.  Any other input (zero, subnormal, huge, infinite, NaN) takes the
+// IEEE division itself.
+__device__ __forceinline__ float div_by(float a, float d, float y, bool dok) {
+    const uint32_t ua = __float_as_uint(a) & 0x7fffffffu;
+    const bool ok = dok & ((ua - 0x21800000u) <= (0x5d800000u - 0x21800000u));
+    const float q0 = a * y;
+    float q = __builtin_fmaf(__builtin_fmaf(-q0, d, a), y, q0);
+    if (__builtin_expect(!ok, 0)) q = a / d;
+    return q;
+}
+
+__device__ __forceinline__ bool div_range_ok(float d) {
+    const uint32_t ud = __float_as_uint(d) & 0x7fffffffu;
+    return (ud - 0x21800000u) <= (0x5d800000u - 0x21800000u);
+}
+
 // dot, length, normalize in GLM 1.0.0's order (func_geometric.inl:48-54,88;
 // func_exponential.inl:136-139).
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
@@ -94,9 +116,11 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     const float ddx = 1.0f / fabsf(dx + 0.000001f);
     const float ddy = 1.0f / fabsf(dy + 0.000001f);
     const float ddz = 1.0f / fabsf(dz + 0.000001f);
-    float ix = ((float)(cx + fx) - sx) / dx;
-    float iy = ((float)(cy + fy) - sy) / dy;
-    float iz = ((float)(cz + fz) - sz) / dz;
+    const float rx = 1.0f / dx, ry = 1.0f / dy, rz = 1.0f / dz;
+    const bool okx = div_range_ok(dx), oky = div_range_ok(dy), okz = div_range_ok(dz);
+    float ix = div_by((float)(cx + fx) - sx, dx, rx, okx);
+    float iy = div_by((float)(cy + fy) - sy, dy, ry, oky);
+    float iz = div_by((float)(cz + fz) - sz, dz, rz, okz);
     const T sdx = IX::delta(g, stx, sty, stz, 0);
     const T sdy = IX::delta(g, stx, sty, stz, 1);
     const T sdz = IX::delta(g, stx, sty, stz, 2);
@@ -143,9 +167,9 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             cx = f2i(sx);
             cy = f2i(sy);
             cz = f2i(sz);
-            ix = ((float)(cx + fx) - sx) / dx;
-            iy = ((float)(cy + fy) - sy) / dy;
-            iz = ((float)(cz + fz) - sz) / dz;
+            ix = div_by((float)(cx + fx) - sx, dx, rx, okx);
+            iy = div_by((float)(cy + fy) - sy, dy, ry, oky);
+            iz = div_by((float)(cz + fz) - sz, dz, rz, okz);
             idx = IX::make(g, cx, cy, cz);
         }
     }

@@ -37,6 +37,13 @@ struct FrameDesc {
     float render_lim;   // (float)render_dist
 };
 
+// Ray/step counters: kCounterSlots slots of one 128-byte line each
+// (u64 rays, DDA steps, pixels); a wave adds into slot (its wave id mod
+// kCounterSlots), so waves rarely share a line and the atomics never queue
+// on one address.  Stats are the sum over slots.
+constexpr int kCounterSlots = 4096;
+constexpr int kCounterStride = 16;  // u64 per slot
+
 // Per-view work description for the persistent batch kernel.
 struct ViewDesc {
     FrameDesc frame;
@@ -65,6 +72,7 @@ int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_
                         int64_t total_tiles, const vrt_batch_config& cfg, int32_t n_workgroups,
                         unsigned int* queue, unsigned long long* counters, hipStream_t s);
 int launch_queue_reset(unsigned int* queue, hipStream_t s);
+int selftest_division(uint32_t d_begin, uint32_t d_count, unsigned long long* bad_dev, hipStream_t s);
 int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu);
 
 int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, int32_t y0, int32_t z0,

@@ -12,6 +12,8 @@
 // The per-pixel work is src/fshader.glsl main() (vrt_device.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "vrt_device.h"
 #include "vrt_internal.h"
 
@@ -33,6 +35,7 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
     return start + local;
 }
 
+// Adds this wave's totals into its counter slot (see kCounterSlots).
 __device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long rays,
                                                unsigned long long iters, unsigned long long pixels) {
     if (!counters) return;
@@ -40,9 +43,11 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
     iters = dev::wave_sum(iters);
     pixels = dev::wave_sum(pixels);
     if ((threadIdx.x & (kWave - 1)) == 0) {
-        atomicAdd(counters + 0, rays);
-        atomicAdd(counters + 1, iters);
-        atomicAdd(counters + 2, pixels);
+        const uint32_t wave = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+        unsigned long long* c = counters + (size_t)(wave % kCounterSlots) * kCounterStride;
+        atomicAdd(c + 0, rays);
+        atomicAdd(c + 1, iters);
+        atomicAdd(c + 2, pixels);
     }
 }
 
@@ -147,6 +152,24 @@ __global__ void __launch_bounds__(256) k_render_batch(GridDesc g, const ViewDesc
     flush_counters(counters, rays, iters, pixels);
 }
 
+// div_by against IEEE division for a = 1 + i/2^23 (all i, one per thread)
+// and d = 1 + j/2^23 for j in [d_begin, d_end).
+__global__ void __launch_bounds__(256) k_selftest_div(uint32_t d_begin, uint32_t d_end,
+                                                       unsigned long long* bad) {
+    const uint32_t ai = blockIdx.x * 256u + threadIdx.x;
+    const float a = __uint_as_float(0x3f800000u | ai);
+    uint32_t n = 0;
+    const uint32_t j0 = d_begin + blockIdx.y * 64u;
+    for (uint32_t k = 0; k < 64u; k++) {
+        const uint32_t j = j0 + k;
+        if (j >= d_end) break;
+        const float d = __uint_as_float(0x3f800000u | j);
+        const float y = 1.0f / d;
+        n += __float_as_uint(dev::div_by(a, d, y, true)) != __float_as_uint(a / d);
+    }
+    if (n) atomicAdd(bad, (unsigned long long)n);
+}
+
 __global__ void k_queue_reset(unsigned int* q) {
     if (threadIdx.x == 0) q[0] = 0u;
 }
@@ -204,6 +227,17 @@ int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_
         hipLaunchKernelGGL(k_render_batch<dev::Idx64>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,
                            n_views, total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
     VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int selftest_division(uint32_t d_begin, uint32_t d_count, unsigned long long* bad_dev, hipStream_t s) {
+    const uint32_t end = (uint32_t)std::min<uint64_t>((uint64_t)d_begin + d_count, 1u << 23);
+    for (uint32_t j = d_begin; j < end; j += 64u * 4096u) {
+        const uint32_t e = std::min<uint32_t>(end, j + 64u * 4096u);
+        dim3 grid((1u << 23) / 256u, (e - j + 63u) / 64u);
+        hipLaunchKernelGGL(k_selftest_div, grid, dim3(256), 0, s, j, e, bad_dev);
+        VRT_HIP(hipGetLastError());
+    }
     return VRT_OK;
 }
 

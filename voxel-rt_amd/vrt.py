@@ -132,6 +132,7 @@ SIGNATURES = {
     "vrt_server_release": (C.c_int, [P, U64]),
     "vrt_server_status": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(U64)]),
     "vrt_server_clock": (C.c_double, [P]),
+    "vrt_selftest_division": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(U64)]),
     "vrt_world_generate": (C.c_int, [C.POINTER(vrt_world_params), P, I32]),
     "vrt_world_generate_device": (C.c_int, [P, C.POINTER(vrt_world_params)]),
     "vrt_world_surface": (C.c_int, [C.POINTER(vrt_world_params), I64, I64, C.POINTER(I32)]),
@@ -411,6 +412,11 @@ class Context:
         s = vrt_stats()
         call("vrt_read_stats", self.h, C.byref(s), 1 if reset else 0)
         return {"rays": s.rays, "dda_steps": s.dda_steps, "pixels": s.pixels, "draws": s.draws}
+
+    def selftest_division(self, d_begin, d_count):
+        bad = U64(0)
+        call("vrt_selftest_division", self.h, int(d_begin), int(d_count), C.byref(bad))
+        return bad.value
 
     def cast_rays(self, start, direction, dist, render_dist):
         start = np.ascontiguousarray(start, dtype=np.float32).reshape(-1, 3)
