@@ -102,6 +102,160 @@ def plain_bench(vrt, steps, warmup):
     return sc, ctx, out
 
 
+def world_ctx(vrt, name, seed):
+    cfg = vrt.CONFIGS[name]
+    dims = cfg["dims"]
+    params = vrt.world_params(dims, seed, caves=cfg["caves"], boxes=cfg["boxes"])
+    world = vrt.generate_world(params)
+    ctx = vrt.Context(dims)
+    ctx.upload(world)
+    ctx.build_depth_field()
+    ctx.sync()
+    return params, dims, world, ctx
+
+
+def percentiles(xs):
+    return {"p50": float(np.percentile(xs, 50)), "p99": float(np.percentile(xs, 99)), "max": float(np.max(xs))}
+
+
+def batched_bench(vrt, reps=5):
+    """64 views of resolution uniform in {320x180 .. 3840x2160}, random poses
+    over a 1024x256x1024 world, 0-16 lights each: 64 plain launches vs one
+    persistent launch (autotuned tile shape and queue chunk, hipGraph)."""
+    C = vrt.C
+    params, dims, world, ctx = world_ctx(vrt, "c4", 4)
+    views = vrt.make_views(params, dims, world, 44, 64)
+    del world
+    targets = [vrt.Target(w, h, depth=True) for (_, w, h) in views]
+    arr = (vrt.vrt_view * len(views))()
+    for i, ((f, w, h), t) in enumerate(zip(views, targets)):
+        arr[i].frame = f
+        arr[i].target = t.struct()
+    stream = ctx.stream
+    pixels = sum(w * h for (_, w, h) in views)
+    # plain: one launch per view, back to back; per-view completion times
+    vrt.call("vrt_draw_views_plain", ctx.h, arr, len(views))
+    ctx.sync()
+    ctx.stats(reset=True)
+    plain_ms, plain_done = [], []
+    for _ in range(reps):
+        evs = [vrt.Event() for _ in range(len(views) + 1)]
+        evs[0].record(stream)
+        for i in range(len(views)):
+            vrt.call("vrt_draw_frame", ctx.h, C.byref(arr[i].frame), C.byref(arr[i].target))
+            evs[i + 1].record(stream)
+        ctx.sync()
+        plain_ms.append(evs[-1].ms_since(evs[0]))
+        plain_done.append([evs[i + 1].ms_since(evs[0]) for i in range(len(views))])
+    st = ctx.stats(reset=True)
+    rays = st["rays"] // reps
+    steps = st["dda_steps"] // reps
+    # autotune, then the tuned persistent launch as a graph
+    best = vrt.vrt_batch_config()
+    res = (vrt.vrt_tune_entry * 32)()
+    n = C.c_int32(0)
+    t0 = time.time()
+    vrt.call("vrt_autotune_batch", ctx.h, arr, len(views), 3, C.byref(best), res, 32, C.byref(n))
+    tune_s = time.time() - t0
+    b = C.c_void_p()
+    vrt.call("vrt_batch_create", ctx.h, arr, len(views), C.byref(best), C.byref(b))
+    vrt.call("vrt_batch_launch", b)
+    ctx.sync()
+    ctx.stats(reset=True)
+    batch_ms = []
+    for _ in range(reps):
+        e0, e1 = vrt.Event(), vrt.Event()
+        e0.record(stream)
+        vrt.call("vrt_batch_launch", b)
+        e1.record(stream)
+        ctx.sync()
+        batch_ms.append(e1.ms_since(e0))
+    st2 = ctx.stats(reset=True)
+    assert st2["rays"] // reps == rays, "batched and plain cast different ray counts"
+    vrt.call("vrt_batch_destroy", b)
+    ctx.close()
+    pm, bm = float(np.median(plain_ms)), float(np.median(batch_ms))
+    done = np.median(np.array(plain_done), axis=0)
+    return {
+        "workload": "64 views, resolutions uniform in {320x180,640x360,1280x720,1920x1080,3840x2160}, "
+                    "random poses over a 1024x256x1024 grid (1 GiB), 0-16 lights per view",
+        "pixels": int(pixels), "rays": int(rays), "dda_steps": int(steps),
+        "plain_ms": pm, "batched_ms": bm, "speedup": pm / bm,
+        "plain_mrays_s": rays / (pm * 1e-3) / 1e6, "batched_mrays_s": rays / (bm * 1e-3) / 1e6,
+        "frame_latency_ms": {"plain": percentiles(done), "batched": percentiles([bm] * len(views))},
+        "tuned": {"tile": [best.tile_w, best.tile_h], "chunk_tiles": best.chunk_tiles, "tune_s": tune_s,
+                  "sweep": [{"tile": [res[i].config.tile_w, res[i].config.tile_h],
+                             "chunk": res[i].config.chunk_tiles, "ms": res[i].ms} for i in range(n.value)]},
+    }
+
+
+def served_bench(vrt, n_requests=240, load=0.7):
+    """Resident 1024x512x1024 world; Poisson render requests (resolutions as
+    in the batched set) at `load` x the plain throughput, 10 removeSphere edits
+    (radius 1-8 at surface points) per 100 requests, 32 GB framebuffer budget."""
+    C = vrt.C
+    params, dims, world, ctx = world_ctx(vrt, "c5", 5)
+    X, Y, Z = dims
+    views = vrt.make_views(params, dims, world, 55, n_requests)
+    del world
+    rng = np.random.default_rng(56)
+    # offered rate from a plain pass over a sample of the requests
+    probe = views[:16]
+    tg = [vrt.Target(w, h, depth=True) for (_, w, h) in probe]
+    ctx.sync()
+    e0, e1 = vrt.Event(), vrt.Event()
+    e0.record(ctx.stream)
+    for (f, w, h), t in zip(probe, tg):
+        ctx.draw(t, frame=f)
+    e1.record(ctx.stream)
+    ctx.sync()
+    mean_ms = e1.ms_since(e0) / len(probe)
+    del tg
+    rate = load / (mean_ms * 1e-3)  # requests per second
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    cfg = vrt.vrt_server_config()
+    cfg.framebuffer_budget = 32 << 30
+    cfg.max_batch_views = 16
+    cfg.streams = 3
+    cfg.want_depth = 1
+    cfg.want_aux = 0
+    s = C.c_void_p()
+    vrt.call("vrt_server_create", ctx.h, C.byref(cfg), C.byref(s))
+    ctx.stats(reset=True)
+    res = (vrt.vrt_result * 64)()
+    n = C.c_int32(0)
+    lat, edits, nxt, done = [], 0, 0, 0
+    t_start = vrt.lib().vrt_server_clock(s)
+    while done < n_requests:
+        now = vrt.lib().vrt_server_clock(s) - t_start
+        while nxt < n_requests and arrivals[nxt] <= now:
+            if nxt % 10 == 9:  # 10 edits per 100 requests
+                x, z = int(rng.integers(0, X)), int(rng.integers(0, Z))
+                y = vrt.surface(params, x, z)
+                vrt.call("vrt_server_submit_edit", s, x, y, z, int(rng.integers(1, 9)), C.byref(C.c_uint64()))
+                edits += 1
+            f, w, h = views[nxt]
+            vrt.call("vrt_server_submit", s, C.byref(f), w, h, float(t_start + arrivals[nxt]), C.byref(C.c_uint64()))
+            nxt += 1
+        vrt.call("vrt_server_pump", s, 0)
+        vrt.call("vrt_server_poll", s, res, 64, C.byref(n))
+        for i in range(n.value):
+            lat.append((res[i].t_done - res[i].t_submit) * 1e3)
+            vrt.call("vrt_server_release", s, res[i].id)
+        done += n.value
+    wall = vrt.lib().vrt_server_clock(s) - t_start
+    st = ctx.stats(reset=True)
+    vrt.call("vrt_server_destroy", s)
+    ctx.close()
+    return {
+        "workload": "resident 1024x512x1024 grid (2 GiB), %d Poisson requests at %.0f req/s (%.0f%% of the plain "
+                    "rate), resolutions as the batched set, %d removeSphere edits (r 1-8), 32 GB framebuffer budget"
+                    % (n_requests, rate, 100 * load, edits),
+        "latency_ms": percentiles(lat), "mrays_s": st["rays"] / wall / 1e6, "rays": int(st["rays"]),
+        "wall_s": wall, "plain_ms_per_request": mean_ms,
+    }
+
+
 def cpu_baseline(vrt, sc, ctx):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as orc  # the oracle, timed as the CPU baseline only
@@ -126,12 +280,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--only", choices=["batched", "served"], default=None,
+                    help="run only this leg (for profiling); prints its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders on a single MI355X (no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     vrt = load()
+    if args.only == "batched":
+        print(json.dumps(batched_bench(vrt, reps=2)), flush=True)
+        return
+    if args.only == "served":
+        print(json.dumps(served_bench(vrt)), flush=True)
+        return
     sc, ctx, plain = plain_bench(vrt, args.steps, args.warmup)
     log("plain: %.1f Mrays/s, %.3f ms/frame" % (plain["value"], plain["kernel_ms"]))
     res = {
@@ -150,6 +312,14 @@ def main():
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(vrt, sc, ctx)
     ctx.close()
+    del sc
+    if not args.no_extra:
+        res["batched"] = batched_bench(vrt)
+        log("batched: plain %.2f ms, batched %.2f ms (x%.2f)" % (
+            res["batched"]["plain_ms"], res["batched"]["batched_ms"], res["batched"]["speedup"]))
+        res["served"] = served_bench(vrt)
+        log("served: p50 %.2f ms p99 %.2f ms, %.0f Mrays/s" % (
+            res["served"]["latency_ms"]["p50"], res["served"]["latency_ms"]["p99"], res["served"]["mrays_s"]))
     print(json.dumps(res), flush=True)
 
 

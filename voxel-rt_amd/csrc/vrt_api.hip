@@ -66,10 +66,27 @@ std::vector<int32_t> depth_table_offsets() {
     return out;
 }
 
+// Largest float t with sqrtf(t) <= r (IEEE sqrt, correctly rounded like the
+// device's); NaN for NaN r, -inf when no t qualifies (r < 0).
+float sqrt_le_limit(float r) {
+    if (r != r) return r;
+    if (r < 0.0f) return -INFINITY;
+    if (r == INFINITY) return INFINITY;
+    float t = r * r;
+    while (t > 0.0f && !(std::sqrt(t) <= r)) t = std::nextafter(t, -INFINITY);
+    for (;;) {
+        const float up = std::nextafter(t, INFINITY);
+        if (!(std::sqrt(up) <= r)) break;
+        t = up;
+    }
+    return t;
+}
+
 FrameDesc make_frame_desc(const vrt_frame& f) {
     FrameDesc d;
     d.f = f;
     d.render_lim = (float)f.render_dist;
+    for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) d.light_dd_max[i] = sqrt_le_limit(f.lights[i].radius);
     return d;
 }
 

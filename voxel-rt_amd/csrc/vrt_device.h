@@ -50,6 +50,25 @@ __device__ __forceinline__ float div_by(float a, float d, float y, bool dok) {
     return q;
 }
 
+// The three quotients of one intersect update share one range test and one
+// fallback branch.
+__device__ __forceinline__ void div3_by(float ax, float ay, float az, float dx, float dy, float dz, float rx,
+                                        float ry, float rz, bool dok, float& qx, float& qy, float& qz) {
+    const float lo = fminf(fminf(fabsf(ax), fabsf(ay)), fabsf(az));
+    const float hi = fmaxf(fmaxf(fabsf(ax), fabsf(ay)), fabsf(az));
+    // NaN numerators may pass (fmin/fmax drop NaN); both paths then give NaN.
+    const bool ok = dok & (lo >= 0x1p-60f) & (hi <= 0x1p60f);
+    const float q0x = ax * rx, q0y = ay * ry, q0z = az * rz;
+    qx = __builtin_fmaf(__builtin_fmaf(-q0x, dx, ax), rx, q0x);
+    qy = __builtin_fmaf(__builtin_fmaf(-q0y, dy, ay), ry, q0y);
+    qz = __builtin_fmaf(__builtin_fmaf(-q0z, dz, az), rz, q0z);
+    if (__builtin_expect(!ok, 0)) {
+        qx = ax / dx;
+        qy = ay / dy;
+        qz = az / dz;
+    }
+}
+
 __device__ __forceinline__ bool div_range_ok(float d) {
     const uint32_t ud = __float_as_uint(d) & 0x7fffffffu;
     return (ud - 0x21800000u) <= (0x5d800000u - 0x21800000u);
@@ -117,10 +136,10 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     const float ddy = 1.0f / fabsf(dy + 0.000001f);
     const float ddz = 1.0f / fabsf(dz + 0.000001f);
     const float rx = 1.0f / dx, ry = 1.0f / dy, rz = 1.0f / dz;
-    const bool okx = div_range_ok(dx), oky = div_range_ok(dy), okz = div_range_ok(dz);
-    float ix = div_by((float)(cx + fx) - sx, dx, rx, okx);
-    float iy = div_by((float)(cy + fy) - sy, dy, ry, oky);
-    float iz = div_by((float)(cz + fz) - sz, dz, rz, okz);
+    const bool dok = div_range_ok(dx) & div_range_ok(dy) & div_range_ok(dz);
+    float ix, iy, iz;
+    div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok, ix, iy,
+            iz);
     const T sdx = IX::delta(g, stx, sty, stz, 0);
     const T sdy = IX::delta(g, stx, sty, stz, 1);
     const T sdz = IX::delta(g, stx, sty, stz, 2);
@@ -167,9 +186,8 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             cx = f2i(sx);
             cy = f2i(sy);
             cz = f2i(sz);
-            ix = div_by((float)(cx + fx) - sx, dx, rx, okx);
-            iy = div_by((float)(cy + fy) - sy, dy, ry, oky);
-            iz = div_by((float)(cz + fz) - sz, dz, rz, okz);
+            div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok,
+                    ix, iy, iz);
             idx = IX::make(g, cx, cy, cz);
         }
     }
@@ -271,8 +289,10 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             }
             if (!((L.pos[0] >= 0.0f) & (L.pos[1] >= 0.0f) & (L.pos[2] >= 0.0f))) continue;
             const float vx0 = L.pos[0] - hx, vy0 = L.pos[1] - hy, vz0 = L.pos[2] - hz;
-            const float dist = sqrtf(dot3(vx0, vy0, vz0, vx0, vy0, vz0));
-            if (!(dist <= L.radius)) continue;
+            const float dd = dot3(vx0, vy0, vz0, vx0, vy0, vz0);
+            // length <= radius (:171), decided without the square root
+            if (!(dd <= F.light_dd_max[i])) continue;
+            const float dist = sqrtf(dd);
             const float inv = 1.0f / dist;
             const float ux1 = vx0 * inv, uy1 = vy0 * inv, uz1 = vz0 * inv;
             float lim = (float)f2i(dist + 1.0f);

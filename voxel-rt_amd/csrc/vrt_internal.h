@@ -35,6 +35,9 @@ struct GridDesc {
 struct FrameDesc {
     vrt_frame f;
     float render_lim;   // (float)render_dist
+    // Per light: the largest float t with sqrt_rn(t) <= radius, so that
+    // length(v) <= radius  <=>  dot(v, v) <= t  (sqrt_rn is monotone).
+    float light_dd_max[VRT_MAX_LOCAL_LIGHTS];
 };
 
 // Ray/step counters: kCounterSlots slots of one 128-byte line each
