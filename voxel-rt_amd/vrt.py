@@ -514,7 +514,7 @@ def surface(params, x, z):
 CONFIGS = {
     "c1": dict(dims=(128, 128, 128), caves=32, boxes=0, size=(256, 256), lights=0, render_dist=384),
     "c2": dict(dims=(512, 256, 512), caves=512, boxes=2000, size=(1920, 1080), lights=16, render_dist=384),
-    "c3": dict(dims=(2048, 512, 2048), caves=110000, boxes=0, size=(3840, 2160), lights=16, render_dist=4096),
+    "c3": dict(dims=(2048, 512, 2048), caves=70000, boxes=0, size=(3840, 2160), lights=16, render_dist=4096),
     "c4": dict(dims=(1024, 256, 1024), caves=2048, boxes=0, size=None, lights=None, render_dist=384),
     "c5": dict(dims=(1024, 512, 1024), caves=2048, boxes=0, size=None, lights=None, render_dist=384),
     # small scenes for smoke and quick parity runs
