@@ -39,6 +39,19 @@ def load():
     return vrt
 
 
+def profiled_traffic():
+    """HBM bytes per k_render_plain launch measured with rocprofv3 --pmc
+    (calibrated FETCH_SIZE + WRITE_SIZE), from the committed counter summary."""
+    path = os.path.join(ROOT, "profiles", "k_render_plain_c2_1080p_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def grid_ptr(vrt, ctx):
     p = vrt.P()
     vrt.call("vrt_grid_device_ptr", ctx.h, vrt.C.byref(p))
@@ -96,7 +109,7 @@ def plain_bench(vrt, steps, warmup):
         "kernel_ms_p50": float(np.percentile(per, 50)), "kernel_ms_p99": float(np.percentile(per, 99)),
         "rays_per_frame": int(rays_per_frame), "dda_steps_per_frame": int(steps_per_frame),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
                      "bytes_per_launch": int(bytes_per_launch), "kernel": "k_render_plain"},
     }
     return sc, ctx, out
