@@ -312,6 +312,10 @@ typedef struct vrt_world_params {
 int vrt_world_generate(const vrt_world_params* p, int32_t* host_out, int32_t threads);
 /* Same world, generated on the device straight into the context's grid. */
 int vrt_world_generate_device(vrt_ctx* ctx, const vrt_world_params* p);
+/* The reference's own world: initVoxels with placeTrunk / placeBush
+ * (src/level.cpp:4-27,59-138) on an X x Y x Z host grid, quirks included;
+ * the reference is X = Z = 512, Y = 96. */
+int vrt_world_reference(int64_t X, int64_t Y, int64_t Z, int32_t* host_out);
 /* Terrain surface height (first empty y of the column before caves/boxes). */
 int vrt_world_surface(const vrt_world_params* p, int64_t x, int64_t z, int32_t* height);
 

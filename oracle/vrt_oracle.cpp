@@ -518,3 +518,99 @@ void orc_primary_dir(const vrt_frame* f, float vx, float vy, float* out) {
         out[c] = (M[0 + c] * d[0] + M[4 + c] * d[1]) + (M[8 + c] * d[2] + M[12 + c] * 0.0f);
 }
 }
+
+/* initVoxels, placeTrunk, placeBush: src/level.cpp:4-27,59-138, transcribed
+ * with VOXELS_WIDTH -> X (x loops) / Z (z loops) and VOXELS_HEIGHT -> Y. */
+namespace {
+struct RefCtx {
+    int32_t* v;
+    int64_t X, Y, Z;
+};
+
+void ref_place_bush(RefCtx& c, int px, int py, int pz, int cr, int cg, int cb, int radius) {
+    for (int z = -radius; z < radius; z++) {
+        for (int y = -radius; y < radius; y++) {
+            for (int x = -radius; x < radius; x++) {
+                if (x + px < c.X && y + py < c.Y && z + pz < c.Z && x + px >= 0 && y + py >= 0 && z + pz >= 0) {
+                    if (x * x + y * y + z * z < radius * radius) {
+                        orc_destroy_voxel(c.v, c.X, c.Y, c.Z, x, y, z);
+                        int voxel = 0;
+                        voxel += cr;
+                        voxel = voxel << 8;
+                        voxel += cg - ((x + y + z) % 3) * 20;
+                        voxel = voxel << 8;
+                        voxel += cb;
+                        orc_place_voxel(c.v, c.X, c.Y, c.Z, x + px, y + py, z + pz, voxel);
+                    }
+                }
+            }
+        }
+    }
+}
+
+void ref_place_trunk(RefCtx& c, int px, int py, int pz, int cr, int cg, int cb, int length) {
+    for (int z = -length >> 2; z < length >> 2; z++) {
+        for (int y = 0; y < length; y++) {
+            for (int x = -length >> 2; x < length >> 2; x++) {
+                if (x + px < c.X && y + py < c.Y && z + pz < c.Z) {
+                    orc_destroy_voxel(c.v, c.X, c.Y, c.Z, x, y, z);
+                    int voxel = 0;
+                    voxel += cr - ((x + z) % 2) * 10;
+                    voxel = voxel << 8;
+                    voxel += cg - ((x + z) % 2) * 10;
+                    voxel = voxel << 8;
+                    voxel += cb;
+                    orc_place_voxel(c.v, c.X, c.Y, c.Z, x + px, y + py, z + pz, voxel);
+                }
+            }
+        }
+    }
+}
+}  // namespace
+
+extern "C" void orc_world_reference(int32_t* vox, int64_t X, int64_t Y, int64_t Z) {
+    RefCtx c{vox, X, Y, Z};
+    for (int64_t i = 0; i < X * Y * Z; i++) vox[i] = -1;  /* initRender, src/render.cpp:349-351 */
+    int voxel;
+    for (int z = 0; z < Z; z++) {
+        for (int y = 0; y < Y; y++) {
+            for (int x = 0; x < X; x++) {
+                int colorVariation = 5 * ((x + y + z) % 3);
+                orc_destroy_voxel(vox, X, Y, Z, x, y, z);
+                if (y <= 25) {
+                    voxel = 0;
+                    voxel += 90 + colorVariation;
+                    voxel = voxel << 8;
+                    voxel += 90 + colorVariation;
+                    voxel = voxel << 8;
+                    voxel += 90 + colorVariation;
+                    orc_place_voxel(vox, X, Y, Z, x, y, z, voxel);
+                } else if (y <= 33) {
+                    voxel = 0;
+                    voxel += 120 + colorVariation;
+                    voxel = voxel << 8;
+                    voxel += 100 + colorVariation;
+                    voxel = voxel << 8;
+                    voxel += 0;
+                    orc_place_voxel(vox, X, Y, Z, x, y, z, voxel);
+                } else if (y <= 36) {
+                    voxel = 0;
+                    voxel += 10;
+                    voxel = voxel << 8;
+                    voxel += 130 + colorVariation;
+                    voxel = voxel << 8;
+                    voxel += 10;
+                    orc_place_voxel(vox, X, Y, Z, x, y, z, voxel);
+                }
+            }
+        }
+    }
+    for (int z = 10; z < Z - 10; z++) {
+        for (int x = 10; x < X - 10; x++) {
+            if (x % 30 == 0 && z % 25 == 0) {
+                ref_place_trunk(c, x + 1 + z % 7, 36, z, 128, 100, 15, 6);
+                ref_place_bush(c, x + z % 7, 36 + 10, z, 15, 128, 15, 6);
+            }
+        }
+    }
+}

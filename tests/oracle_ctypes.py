@@ -43,6 +43,7 @@ def lib():
         L.orc_dot.argtypes = [P, P]
         L.orc_dot.restype = C.c_float
         L.orc_primary_dir.argtypes = [P, C.c_float, C.c_float, P]
+        L.orc_world_reference.argtypes = [P, i64, i64, i64]
         _lib = L
     return _lib
 
@@ -124,6 +125,13 @@ def render(world_df, dims, frame, width, height, pixels=None, nthreads=None):
     out["rays"] = int(totals[0])
     out["dda_steps"] = int(totals[1])
     return out
+
+
+def world_reference(dims=(512, 96, 512)):
+    """initVoxels' world (src/level.cpp:82-138)."""
+    w = np.empty(dims[0] * dims[1] * dims[2], np.int32)
+    lib().orc_world_reference(_p(w), dims[0], dims[1], dims[2])
+    return w
 
 
 def normalize(v):

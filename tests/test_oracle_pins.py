@@ -415,3 +415,26 @@ def test_remove_sphere_equals_rebuild_inside_repair_radius(orc):
     for k in np.nonzero(before < 0)[0]:
         assert jump(df[k]) >= jump(before[k])
     assert np.all((df >= 0) <= (before >= 0))  # solids only disappear
+
+
+# ------------------------------------------------------------ reference world
+@pytest.mark.parametrize("dims", [(512, 96, 512), (200, 60, 130), (25, 40, 31)])
+def test_reference_world_product_equals_oracle(orc, vrt, dims):
+    a = orc.world_reference(dims)
+    b = vrt.reference_world(dims)
+    assert np.array_equal(a, b), "differs at %d voxels" % int((a != b).sum())
+
+
+def test_reference_world_closed_forms(orc):
+    dims = (512, 96, 512)
+    w = orc.world_reference(dims).reshape(512, 96, 512)  # [z][y][x]
+    # layers (src/level.cpp:93-125) away from trees and the origin quirk
+    assert w[300, 10, 301] == ((90 + 5 * ((301 + 10 + 300) % 3)) * 0x010101)
+    assert w[300, 30, 301] == (((120 + 5 * ((301 + 30 + 300) % 3)) << 16) | ((100 + 5 * ((301 + 30 + 300) % 3)) << 8))
+    assert w[300, 36, 301] == ((10 << 16) | ((130 + 5 * ((301 + 36 + 300) % 3)) << 8) | 10)
+    assert np.all(w[:, 37:, :][w[:, 37:, :] >= 0] > 0)  # only trees above the grass
+    # 16 x 19 trees: x in 30..480 step 30, z in 25..500 step 25
+    assert ((w[:, 40, :] >= 0).sum()) > 16 * 19 * 9
+    # destroyVoxel with local offsets clears cells at the origin (quirk kept)
+    assert w[0, 0, 0] == -1 and w[0, 5, 0] == -1 and w[0, 0, 5] == -1 and w[5, 0, 0] == -1
+    assert w[0, 0, 6] >= 0 and w[0, 6, 0] >= 0

@@ -141,6 +141,75 @@ int make_world_plan(const vrt_world_params& p, WorldPlan& plan) {
 
 using namespace vrt;
 
+// The reference's own world, initVoxels (src/level.cpp:82-138) for an
+// X x Y x Z grid (reference 512 x 96 x 512): stone up to y 25, dirt to 33,
+// grass to 36 with a colour variation of 5*((x+y+z)%3); then a tree at every
+// (x, z) in [10, X-10) x [10, Z-10) with x%30 == 0 and z%25 == 0: a 6-high
+// trunk at (x+1+z%7, 36, z) and a radius-6 bush at (x+z%7, 46, z).  Kept
+// quirks: placeTrunk's extent is (-length)>>2 .. length>>2 (:60,62), both
+// helpers call destroyVoxel with LOCAL offsets (:11,64), clearing cells near
+// the origin, and placeTrunk checks only the upper bounds before placeVoxel.
+namespace {
+struct RefWorld {
+    int32_t* v;
+    int64_t X, Y, Z;
+    bool in(int64_t x, int64_t y, int64_t z) const {
+        return x >= 0 && y >= 0 && z >= 0 && x < X && y < Y && z < Z;
+    }
+    void put(int64_t x, int64_t y, int64_t z, int32_t w) {
+        if (in(x, y, z)) v[x + X * y + X * Y * z] = w;
+    }
+};
+
+int32_t rgb(int32_t r, int32_t g, int32_t b) { return (((r << 8) + g) << 8) + b; }
+
+void ref_trunk(RefWorld& w, int64_t px, int64_t py, int64_t pz) {
+    const int length = 6;
+    for (int z = (-length) >> 2; z < (length >> 2); z++)
+        for (int y = 0; y < length; y++)
+            for (int x = (-length) >> 2; x < (length >> 2); x++) {
+                if (!(x + px < w.X && y + py < w.Y && z + pz < w.Z)) continue;
+                w.put(x, y, z, -1);  // destroyVoxel(x, y, z): local offsets
+                const int v = ((x + z) % 2) * 10;
+                w.put(x + px, y + py, z + pz, rgb(128 - v, 100 - v, 15));
+            }
+}
+
+void ref_bush(RefWorld& w, int64_t px, int64_t py, int64_t pz) {
+    const int r = 6;
+    for (int z = -r; z < r; z++)
+        for (int y = -r; y < r; y++)
+            for (int x = -r; x < r; x++) {
+                if (!w.in(x + px, y + py, z + pz) || x * x + y * y + z * z >= r * r) continue;
+                w.put(x, y, z, -1);  // destroyVoxel(x, y, z): local offsets
+                w.put(x + px, y + py, z + pz, rgb(15, 128 - ((x + y + z) % 3) * 20, 15));
+            }
+}
+}  // namespace
+
+extern "C" int vrt_world_reference(int64_t X, int64_t Y, int64_t Z, int32_t* out) {
+    if (!out || X <= 0 || Y <= 0 || Z <= 0 || X > 0x7fffffff || Y > 0x7fffffff || Z > 0x7fffffff)
+        return fail(VRT_ERR_INVALID, "invalid reference world arguments");
+    RefWorld w{out, X, Y, Z};
+    for (int64_t z = 0; z < Z; z++)
+        for (int64_t y = 0; y < Y; y++)
+            for (int64_t x = 0; x < X; x++) {
+                const int32_t cv = (int32_t)(5 * ((x + y + z) % 3));
+                int32_t v = -1;
+                if (y <= 25) v = rgb(90 + cv, 90 + cv, 90 + cv);
+                else if (y <= 33) v = rgb(120 + cv, 100 + cv, 0);
+                else if (y <= 36) v = rgb(10, 130 + cv, 10);
+                out[x + X * y + X * Y * z] = v;
+            }
+    for (int64_t z = 10; z < Z - 10; z++)
+        for (int64_t x = 10; x < X - 10; x++)
+            if (x % 30 == 0 && z % 25 == 0) {
+                ref_trunk(w, x + 1 + z % 7, 36, z);
+                ref_bush(w, x + z % 7, 36 + 10, z);
+            }
+    return VRT_OK;
+}
+
 extern "C" int vrt_world_surface(const vrt_world_params* p, int64_t x, int64_t z, int32_t* height) {
     if (!p || !height) return fail(VRT_ERR_INVALID, "null argument");
     *height = wg::surface_height(x, z, p->seed, p->octaves, p->base_cell, p->height_min, p->height_max);

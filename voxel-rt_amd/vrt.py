@@ -136,6 +136,7 @@ SIGNATURES = {
     "vrt_world_generate": (C.c_int, [C.POINTER(vrt_world_params), P, I32]),
     "vrt_world_generate_device": (C.c_int, [P, C.POINTER(vrt_world_params)]),
     "vrt_world_surface": (C.c_int, [C.POINTER(vrt_world_params), I64, I64, C.POINTER(I32)]),
+    "vrt_world_reference": (C.c_int, [I64, I64, I64, P]),
 }
 
 _lib = None
@@ -228,16 +229,22 @@ class DeviceBuffer:
     def upload(self, arr):
         arr = np.ascontiguousarray(arr)
         assert arr.nbytes <= self.nbytes
+        hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
         hip_check(hip().hipMemcpy(self.ptr, arr.ctypes.data_as(P), arr.nbytes, H2D), "hipMemcpy H2D")
 
     def download(self, dtype, shape):
+        """Copy to the host after all queued device work: hipMemcpy on the null
+        stream does not wait for a context's non-blocking stream."""
         out = np.empty(shape, dtype=dtype)
         assert out.nbytes <= self.nbytes
+        hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
         hip_check(hip().hipMemcpy(out.ctypes.data_as(P), self.ptr, out.nbytes, D2H), "hipMemcpy D2H")
         return out
 
     def fill(self, byte):
+        hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
         hip_check(hip().hipMemset(self.ptr, byte, self.nbytes), "hipMemset")
+        hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
     def free(self):
         if self.ptr:
@@ -323,6 +330,7 @@ def download_target(t, aux=True, depth=True):
 
     def get(ptr, dtype, shape):
         out = np.empty(shape, dtype=dtype)
+        hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
         hip_check(hip().hipMemcpy(out.ctypes.data_as(P), P(ptr), out.nbytes, D2H), "hipMemcpy D2H")
         return out
 
@@ -498,6 +506,13 @@ def generate_world(params, threads=0):
     n = params.size[0] * params.size[1] * params.size[2]
     out = np.empty(n, dtype=np.int32)
     call("vrt_world_generate", C.byref(params), out.ctypes.data_as(P), int(threads))
+    return out
+
+
+def reference_world(dims=(512, 96, 512)):
+    """initVoxels' world (src/level.cpp:82-138), host array."""
+    out = np.empty(dims[0] * dims[1] * dims[2], dtype=np.int32)
+    call("vrt_world_reference", int(dims[0]), int(dims[1]), int(dims[2]), out.ctypes.data_as(P))
     return out
 
 
