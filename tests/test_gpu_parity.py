@@ -212,3 +212,50 @@ def test_device_copy_and_generator(gpu):
         gpu.call("vrt_upload_device", ctx.h, other.addr)
         assert np.array_equal(ctx.download(), host[::-1])
         ctx.close()
+
+
+def degenerate_cases(gpu):
+    rng = np.random.default_rng(12)
+    cases = []
+    for name, dims, fill in [("empty", (40, 30, 50), "empty"), ("solid", (40, 30, 50), "solid"),
+                             ("single", (40, 30, 50), "single"), ("one-cell", (1, 1, 1), "solid"),
+                             ("column", (1, 64, 1), "random"), ("plate", (64, 1, 64), "random"),
+                             ("sparse", (33, 17, 29), "random")]:
+        n = dims[0] * dims[1] * dims[2]
+        if fill == "empty":
+            w = np.full(n, -1, np.int32)
+        elif fill == "solid":
+            w = rng.integers(0, 1 << 24, n).astype(np.int32)
+        elif fill == "single":
+            w = np.full(n, -1, np.int32)
+            w[20 + 40 * 10 + 40 * 30 * 25] = 0x00FF00
+        else:
+            w = np.where(rng.random(n) < 0.1, rng.integers(0, 1 << 24, n), -1).astype(np.int32)
+        cases.append((name, dims, w))
+    return cases
+
+
+@pytest.mark.parametrize("idx", range(7))
+def test_degenerate_worlds_and_ragged_frames(gpu, orc, idx):
+    name, dims, w = degenerate_cases(gpu)[idx]
+    ctx = gpu.Context(dims)
+    ctx.upload(w)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    assert np.array_equal(wdf, orc.depth_field(w, dims)), name + " depth field"
+    rng = np.random.default_rng(idx)
+    for (W, H, rd) in [(13, 7, 384), (1, 1, 384), (64, 40, 0), (50, 50, 5)]:
+        f = gpu.frame_defaults(dims[0], W, H)
+        f.cam_pos[:] = [dims[0] * 0.5 + 0.25, dims[1] * 0.5 + 0.1, dims[2] * 0.5 - 0.3]
+        f.rotate_matrix[:] = gpu.camera_matrix(rng.uniform(-1, 1), rng.uniform(0, 6.28)).tolist()
+        f.render_dist = rd
+        f.lights[0].pos[:] = [dims[0] * 0.5, dims[1] * 0.5, dims[2] * 0.5]
+        f.lights[0].rgb[:] = [1.0, 0.5, 2.0]
+        t = gpu.Target(W, H, depth=True, aux=True)
+        ctx.stats(reset=True)
+        ctx.draw(t, frame=f)
+        g = t.download()
+        g.update(ctx.stats(reset=True))
+        r = orc.render(wdf, dims, f, W, H)
+        assert_frames_match(g, r, "%s %dx%d rd=%d" % (name, W, H, rd))
+    ctx.close()
