@@ -276,6 +276,9 @@ int vrt_server_submit_edit(vrt_server* s, int32_t cx, int32_t cy, int32_t cz, in
 int vrt_server_pump(vrt_server* s, int32_t wait);
 /* Completed requests, oldest first; up to max. */
 int vrt_server_poll(vrt_server* s, vrt_result* out, int32_t max, int32_t* n);
+/* Drops a request that is still queued (not yet admitted); admitted
+ * requests run to completion.  VRT_ERR_NOTFOUND otherwise. */
+int vrt_server_cancel(vrt_server* s, uint64_t id);
 /* Frees a completed request's buffers back to the budget. */
 int vrt_server_release(vrt_server* s, uint64_t id);
 /* Queued, in-flight and completed-unreleased request counts and bytes in use. */

@@ -265,3 +265,63 @@ def test_server_admission_under_budget(gpu):
     assert q.value == 1 and used.value == 3 * per
     gpu.call("vrt_server_destroy", s)
     ctx.close()
+
+
+def test_server_cancel_queued_request(gpu):
+    sc = gpu.make_scene("tiny", seed=3)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    per = 64 * 64 * 7 + 6 * 256
+    s = C.c_void_p()
+    gpu.call("vrt_server_create", ctx.h, C.byref(server_config(gpu, per, max_views=4, aux=0)), C.byref(s))
+    ids = []
+    for _ in range(3):
+        rid = C.c_uint64(0)
+        gpu.call("vrt_server_submit", s, C.byref(sc.frame), 64, 64, 0.0, C.byref(rid))
+        ids.append(rid.value)
+    gpu.call("vrt_server_pump", s, 0)  # the budget admits one
+    assert gpu.lib().vrt_server_cancel(s, ids[0]) == gpu.VRT_ERR_NOTFOUND  # already admitted
+    gpu.call("vrt_server_cancel", s, ids[1])
+    got = drain(gpu, s, 2)
+    assert sorted(got) == [ids[0], ids[2]]
+    q, fl, d, used = C.c_int32(), C.c_int32(), C.c_int32(), C.c_uint64()
+    gpu.call("vrt_server_status", s, C.byref(q), C.byref(fl), C.byref(d), C.byref(used))
+    assert q.value == 0 and fl.value == 0 and used.value == 0
+    gpu.call("vrt_server_destroy", s)
+    ctx.close()
+
+
+def test_failures_are_reported_not_fatal(gpu):
+    """Allocation failure, invalid frames, targets and edits return errors and
+    leave the library usable."""
+    h = C.c_void_p()
+    # 2^20 x 2^20 x 2^10 voxels = 4 PiB: the grid allocation must fail cleanly
+    assert gpu.lib().vrt_create(C.byref(h), 1 << 20, 1 << 20, 1 << 10, 0) == gpu.VRT_ERR_NOMEM and not h.value
+    sc = gpu.make_scene("tiny", seed=5)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    f = sc.frame
+    t = gpu.Target(16, 16)
+    bad = gpu.vrt_frame.from_buffer_copy(f)
+    bad.render_dist = -1
+    assert gpu.lib().vrt_draw_frame(ctx.h, C.byref(bad), C.byref(t.struct())) == gpu.VRT_ERR_INVALID
+    bad.render_dist = (1 << 24) + 1
+    assert gpu.lib().vrt_set_frame(ctx.h, C.byref(bad)) == gpu.VRT_ERR_INVALID
+    ts = t.struct()
+    ts.rgb = None
+    assert gpu.lib().vrt_draw_frame(ctx.h, C.byref(f), C.byref(ts)) == gpu.VRT_ERR_INVALID
+    ts = t.struct()
+    ts.width = 0
+    assert gpu.lib().vrt_draw_frame(ctx.h, C.byref(f), C.byref(ts)) == gpu.VRT_ERR_INVALID
+    assert gpu.lib().vrt_remove_sphere(ctx.h, 5, 5, 5, 100000) == gpu.VRT_ERR_INVALID
+    cfg = gpu.vrt_batch_config(8, 16, 1, 0)  # not a supported tile shape
+    arr = (gpu.vrt_view * 1)()
+    arr[0].frame = f
+    arr[0].target = t.struct()
+    b = C.c_void_p()
+    assert gpu.lib().vrt_batch_create(ctx.h, arr, 1, C.byref(cfg), C.byref(b)) == gpu.VRT_ERR_INVALID
+    # still usable
+    ctx.draw(t, frame=f)
+    assert t.download()["rgb"].shape == (16, 16, 3)
+    ctx.close()

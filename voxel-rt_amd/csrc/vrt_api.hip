@@ -24,6 +24,10 @@ int fail(int code, const std::string& msg) {
 }
 
 int hip_fail(hipError_t e, const char* what) {
+    // consume the runtime's last-error state so that the launch checks of
+    // later calls do not report this failure again
+    (void)hipGetLastError();
+    if (e == hipErrorOutOfMemory) return fail(VRT_ERR_NOMEM, std::string(what) + ": " + hipGetErrorString(e));
     return fail(VRT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
@@ -234,6 +238,7 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
         return bail(hip_fail(e, "hipStreamCreate"));
     c->stream = c->own_stream;
     if (hipMalloc(&c->grid.vox, count * sizeof(int32_t)) != hipSuccess) {
+        (void)hipGetLastError();
         c->grid.vox = nullptr;
         return bail(fail(VRT_ERR_NOMEM, "cannot allocate the voxel grid"));
     }

@@ -361,6 +361,19 @@ int vrt_server_submit_edit(vrt_server* s, int32_t cx, int32_t cy, int32_t cz, in
     return VRT_OK;
 }
 
+int vrt_server_cancel(vrt_server* s, uint64_t id) {
+    if (!s) return fail(VRT_ERR_INVALID, "invalid server");
+    for (auto it = s->fifo.begin(); it != s->fifo.end(); ++it) {
+        if (!it->is_edit && it->id == id) {
+            s->fifo.erase(it);
+            s->reqs.erase(id);
+            s->queued--;
+            return VRT_OK;
+        }
+    }
+    return fail(VRT_ERR_NOTFOUND, "no queued request with this id (admitted requests run to completion)");
+}
+
 int vrt_server_pump(vrt_server* s, int32_t wait) {
     if (!s) return fail(VRT_ERR_INVALID, "invalid server");
     VRT_HIP(hipSetDevice(s->ctx->device));

@@ -130,6 +130,7 @@ SIGNATURES = {
     "vrt_server_pump": (C.c_int, [P, I32]),
     "vrt_server_poll": (C.c_int, [P, C.POINTER(vrt_result), I32, C.POINTER(I32)]),
     "vrt_server_release": (C.c_int, [P, U64]),
+    "vrt_server_cancel": (C.c_int, [P, U64]),
     "vrt_server_status": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(U64)]),
     "vrt_server_clock": (C.c_double, [P]),
     "vrt_selftest_division": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(U64)]),
