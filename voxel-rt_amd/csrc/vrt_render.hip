@@ -104,14 +104,16 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
 // 64: a wave shades 64 pixels per pass (lane -> (lane % tw, lane / tw)) and
 // tw*th/64 passes per tile.  The queue counts tiles; a pop takes `chunk`.
 template <typename IX>
-__global__ void __launch_bounds__(256) k_render_batch(GridDesc g, const ViewDesc* __restrict__ views,
-                                                      int32_t n_views, int64_t total_tiles, int32_t tw,
-                                                      int32_t th, int32_t chunk, unsigned int* queue,
-                                                      unsigned long long* counters) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t tw,
+               int32_t th, int32_t chunk, unsigned int* queue, unsigned long long* counters) {
     const int32_t lane = threadIdx.x & 63;
     const int32_t rows_per_pass = kWave / tw;
     const int32_t passes = th / rows_per_pass;
-    unsigned long long rays = 0, iters = 0, pixels = 0;
+    // per-lane totals stay far below 2^32 (a lane shades at most a few
+    // thousand pixels of at most 17 rays x 2^24 steps... per launch it is
+    // bounded by the tiles it pops); widened when the wave flushes
+    uint32_t rays = 0, iters = 0, pixels = 0;
     int32_t v = 0;  // views are popped in increasing order: search forward
     for (;;) {
         uint32_t base = 0;
