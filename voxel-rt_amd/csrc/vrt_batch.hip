@@ -125,7 +125,7 @@ int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_b
     hipError_t e;
     if ((e = hipMalloc(&b->views_dev, b->views.size() * sizeof(ViewDesc))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc views"));
-    if ((e = hipMalloc(&b->queue, 64)) != hipSuccess) return bail(hip_fail(e, "hipMalloc queue"));
+    if ((e = hipMalloc(&b->queue, kQueueBytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc queue"));
     if ((e = hipMemcpy(b->views_dev, b->views.data(), b->views.size() * sizeof(ViewDesc), hipMemcpyHostToDevice)) !=
         hipSuccess)
         return bail(hip_fail(e, "hipMemcpy views"));

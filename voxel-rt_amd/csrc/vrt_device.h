@@ -136,7 +136,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     const float ddy = 1.0f / fabsf(dy + 0.000001f);
     const float ddz = 1.0f / fabsf(dz + 0.000001f);
     const float rx = 1.0f / dx, ry = 1.0f / dy, rz = 1.0f / dz;
-    const bool dok = div_range_ok(dx) & div_range_ok(dy) & div_range_ok(dz);
+    const bool dok = (int(div_range_ok(dx)) & int(div_range_ok(dy)) & int(div_range_ok(dz))) != 0;
     float ix, iy, iz;
     div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok, ix, iy,
             iz);

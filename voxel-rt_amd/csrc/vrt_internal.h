@@ -47,6 +47,9 @@ struct FrameDesc {
 constexpr int kCounterSlots = 4096;
 constexpr int kCounterStride = 16;  // u64 per slot
 
+// Queue of the persistent batch kernel: 8 counters (one per XCD), 128 B apart.
+constexpr size_t kQueueBytes = 8 * 128;
+
 // Per-view work description for the persistent batch kernel.
 struct ViewDesc {
     FrameDesc frame;

@@ -25,6 +25,7 @@ constexpr int kWave = 64;
 constexpr int kPlainWaves = 4;                 // waves per workgroup
 constexpr int kPlainThreads = kWave * kPlainWaves;
 constexpr int kXcds = 8;
+constexpr int kQueueStride = 32;  // queue counters 128 bytes apart
 
 // Bijection from dispatch order to a logical workgroup id such that the
 // workgroups landing on XCD k (dispatch ids = k mod 8) own a contiguous range.
@@ -102,7 +103,12 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
 
 // Persistent multi-view kernel.  Tile shape tw x th with tw*th a multiple of
 // 64: a wave shades 64 pixels per pass (lane -> (lane % tw, lane / tw)) and
-// tw*th/64 passes per tile.  The queue counts tiles; a pop takes `chunk`.
+// tw*th/64 passes per tile.  The tiles (views largest first, each row-major)
+// are split into kXcds contiguous ranges with one queue counter each, on its
+// own cache line; a wave drains the range of the XCD it runs on (workgroups
+// are dispatched round-robin over the XCDs, so blockIdx.x mod 8) and then
+// steals from the others.  A pop takes `chunk` tiles; every wave leaves once
+// all ranges are empty.
 template <typename IX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t tw,
@@ -110,43 +116,47 @@ k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, 
     const int32_t lane = threadIdx.x & 63;
     const int32_t rows_per_pass = kWave / tw;
     const int32_t passes = th / rows_per_pass;
-    // per-lane totals stay far below 2^32 (a lane shades at most a few
-    // thousand pixels of at most 17 rays x 2^24 steps... per launch it is
-    // bounded by the tiles it pops); widened when the wave flushes
+    // per-lane totals stay far below 2^32 for any launch the host accepts
+    // (tiles per launch < 2^32, 17 rays and at most 17 * 2^24 steps a pixel
+    // would overflow only past ~15 pixels at the step cap; counts are widened
+    // when the wave flushes)
     uint32_t rays = 0, iters = 0, pixels = 0;
-    int32_t v = 0;  // views are popped in increasing order: search forward
-    for (;;) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(queue, (unsigned int)chunk);
-        base = __builtin_amdgcn_readfirstlane(base);
-        if ((int64_t)base >= total_tiles) break;
-        int64_t end = (int64_t)base + chunk;
-        if (end > total_tiles) end = total_tiles;
-        for (int64_t tile = base; tile < end; tile++) {
-            // view owning this tile (uniform across the wave)
-            int32_t lo = 0, hi = n_views - 1;
-            while (lo < hi) {
-                const int32_t mid = (lo + hi + 1) >> 1;
-                if (views[mid].first_tile <= tile) lo = mid;
-                else hi = mid - 1;
-            }
-            v = __builtin_amdgcn_readfirstlane(lo);
-            const ViewDesc& V = views[v];
-            const int32_t local = (int32_t)(tile - V.first_tile);
-            const int32_t tx0 = (local % V.tiles_x) * tw;
-            const int32_t ty0 = (local / V.tiles_x) * th;
-            for (int32_t pass = 0; pass < passes; pass++) {
-                const int32_t px = tx0 + lane % tw;
-                const int32_t row = ty0 + pass * rows_per_pass + lane / tw;
-                if (px < V.target.width && row < V.target.height) {
-                    float vx, vy;
-                    dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
-                    dev::PixelResult p;
-                    dev::shade_pixel<IX>(g, V.frame, vx, vy, p);
-                    dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
-                    rays += p.rays;
-                    iters += p.iters;
-                    pixels += 1;
+    const uint32_t home = blockIdx.x % kXcds;
+    for (uint32_t k = 0; k < kXcds; k++) {
+        const uint32_t q = (home + k) % kXcds;
+        const int64_t r0 = total_tiles * q / kXcds, r1 = total_tiles * (q + 1) / kXcds;
+        for (;;) {
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(queue + q * kQueueStride, (unsigned int)chunk);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if ((int64_t)base >= r1 - r0) break;
+            int64_t end = r0 + (int64_t)base + chunk;
+            if (end > r1) end = r1;
+            for (int64_t tile = r0 + base; tile < end; tile++) {
+                // view owning this tile (uniform across the wave)
+                int32_t lo = 0, hi = n_views - 1;
+                while (lo < hi) {
+                    const int32_t mid = (lo + hi + 1) >> 1;
+                    if (views[mid].first_tile <= tile) lo = mid;
+                    else hi = mid - 1;
+                }
+                const ViewDesc& V = views[__builtin_amdgcn_readfirstlane(lo)];
+                const int32_t local = (int32_t)(tile - V.first_tile);
+                const int32_t tx0 = (local % V.tiles_x) * tw;
+                const int32_t ty0 = (local / V.tiles_x) * th;
+                for (int32_t pass = 0; pass < passes; pass++) {
+                    const int32_t px = tx0 + lane % tw;
+                    const int32_t row = ty0 + pass * rows_per_pass + lane / tw;
+                    if (px < V.target.width && row < V.target.height) {
+                        float vx, vy;
+                        dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
+                        dev::PixelResult p;
+                        dev::shade_pixel<IX>(g, V.frame, vx, vy, p);
+                        dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
+                        rays += p.rays;
+                        iters += p.iters;
+                        pixels += 1;
+                    }
                 }
             }
         }
@@ -173,7 +183,7 @@ __global__ void __launch_bounds__(256) k_selftest_div(uint32_t d_begin, uint32_t
 }
 
 __global__ void k_queue_reset(unsigned int* q) {
-    if (threadIdx.x == 0) q[0] = 0u;
+    if (threadIdx.x < kXcds) q[threadIdx.x * kQueueStride] = 0u;
 }
 
 }  // namespace

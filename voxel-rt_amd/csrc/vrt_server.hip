@@ -309,7 +309,7 @@ int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** o
         if ((e = hipMalloc(&sl.views_dev, vbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc"));
         if ((e = hipHostMalloc(&sl.views_host, vbytes, hipHostMallocDefault)) != hipSuccess)
             return bail(hip_fail(e, "hipHostMalloc"));
-        if ((e = hipMalloc(&sl.queue, 64)) != hipSuccess) return bail(hip_fail(e, "hipMalloc"));
+        if ((e = hipMalloc(&sl.queue, kQueueBytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc"));
     }
     *out = s;
     return VRT_OK;
