@@ -2,9 +2,12 @@
 //
 //   k_render_plain : one frame; a wave shades one 8x8 screen tile (one ray per
 //                    lane), a 256-thread workgroup four horizontally adjacent
-//                    tiles; workgroups are renumbered so that each of the 8
-//                    XCDs (round-robin dispatch) walks a contiguous band of
-//                    the image and its L2 sees neighbouring rays.
+//                    tiles, in row-major order.  The hardware deals
+//                    workgroups to the 8 XCDs round-robin, so every XCD gets
+//                    an even share of sky, horizon and terrain tiles; giving
+//                    each XCD a contiguous band of rows instead (for L2
+//                    locality) made the XCDs' loads unequal and the frame
+//                    29% slower.
 //   k_render_batch : many views in one persistent launch; every wave pops
 //                    chunks of (view, tile) work from a device atomic queue
 //                    until the queue is empty (the exit every wave reaches).
@@ -27,15 +30,6 @@ constexpr int kPlainThreads = kWave * kPlainWaves;
 constexpr int kXcds = 8;
 constexpr int kQueueStride = 32;  // queue counters 128 bytes apart
 
-// Bijection from dispatch order to a logical workgroup id such that the
-// workgroups landing on XCD k (dispatch ids = k mod 8) own a contiguous range.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t bid, uint32_t nb) {
-    const uint32_t xcd = bid % kXcds, local = bid / kXcds;
-    const uint32_t base = nb / kXcds, rem = nb % kXcds;
-    const uint32_t start = xcd * base + (xcd < rem ? xcd : rem);
-    return start + local;
-}
-
 // Adds this wave's totals into its counter slot (see kCounterSlots).
 __device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long rays,
                                                unsigned long long iters, unsigned long long pixels) {
@@ -55,7 +49,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
 template <typename IX>
 __global__ void __launch_bounds__(kPlainThreads) k_render_plain(GridDesc g, FrameDesc F, vrt_target t,
                                                                 unsigned long long* counters) {
-    const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t lb = blockIdx.x;
     const int32_t tiles_x = (t.width + 7) >> 3;
     const int32_t tiles_y = (t.height + 7) >> 3;
     const int32_t tile = (int32_t)lb * kPlainWaves + (int32_t)(threadIdx.x >> 6);
