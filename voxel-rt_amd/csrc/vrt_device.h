@@ -50,14 +50,17 @@ __device__ __forceinline__ float div_by(float a, float d, float y, bool dok) {
     return q;
 }
 
-// The three quotients of one intersect update share one range test and one
-// fallback branch.
+// The three quotients (cell boundary - position) / d of one intersect update
+// share one range test and one fallback branch.  With |s| < 2^31 the cell
+// int(s) does not saturate and every numerator is below 2^33 in magnitude, so
+// only the lower end of the fast path's range is tested.  Larger numerators
+// come only from a saturated cell (|s| >= 2^31 or infinite), which lies outside
+// every grid: the ray's next step then leaves the grid and the quotients are
+// never read.  A NaN numerator gives NaN on both paths.
 __device__ __forceinline__ void div3_by(float ax, float ay, float az, float dx, float dy, float dz, float rx,
                                         float ry, float rz, bool dok, float& qx, float& qy, float& qz) {
     const float lo = fminf(fminf(fabsf(ax), fabsf(ay)), fabsf(az));
-    const float hi = fmaxf(fmaxf(fabsf(ax), fabsf(ay)), fabsf(az));
-    // NaN numerators may pass (fmin/fmax drop NaN); both paths then give NaN.
-    const bool ok = dok & (lo >= 0x1p-60f) & (hi <= 0x1p60f);
+    const bool ok = dok & (lo >= 0x1p-60f);
     const float q0x = ax * rx, q0y = ay * ry, q0z = az * rz;
     qx = __builtin_fmaf(__builtin_fmaf(-q0x, dx, ax), rx, q0x);
     qy = __builtin_fmaf(__builtin_fmaf(-q0y, dy, ay), ry, q0y);
@@ -92,7 +95,7 @@ __device__ __forceinline__ uint8_t unorm8(float c) {
 struct Hit {
     int64_t index;   // voxel index or -1
     float pos[3];    // hitPos (valid on a hit)
-    int32_t axis;    // axis of the last step (-1 if the loop never ran)
+    int32_t axis;    // axis of the step into the hit voxel (-1 on a miss)
     int32_t step;    // step on that axis
     uint32_t iters;  // loop iterations (stepCount increments)
 };
@@ -146,7 +149,6 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     T idx = IX::make(g, cx, cy, cz);
     const int32_t* __restrict__ vox = g.vox;
 
-    float cur = 0.0f;
     float trav = 0.0f;
     uint32_t iters = 0;
     int32_t axis = -1;
@@ -157,38 +159,41 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
         const bool bx = (ix < iy) & (ix < iz);
         const bool by = (iy < ix) & (iy < iz);
         const bool bz = !(bx | by);
-        cur = bx ? ix : (by ? iy : iz);
-        cx = bx ? cx + stx : cx;
-        cy = by ? cy + sty : cy;
-        cz = bz ? cz + stz : cz;
-        ix = bx ? ix + ddx : ix;
-        iy = by ? iy + ddy : iy;
-        iz = bz ? iz + ddz : iz;
+        cx += bx ? stx : 0;
+        cy += by ? sty : 0;
+        cz += bz ? stz : 0;
         idx += bx ? sdx : (by ? sdy : sdz);
-        axis = bx ? 0 : (by ? 1 : 2);
         const bool inb = ((uint32_t)cx < g.X) & ((uint32_t)cy < g.Y) & ((uint32_t)cz < g.Z);
         if (!inb) break;
         const int32_t v = vox[idx];
+        // currentIntersect: the intersect of the axis just stepped, before its update
+        const float cur = bx ? ix : (by ? iy : iz);
         if (v >= 0) {
             h.pos[0] = dx * cur + sx;
             h.pos[1] = dy * cur + sy;
             h.pos[2] = dz * cur + sz;
+            axis = bx ? 0 : (by ? 1 : 2);
             result = (int64_t)idx;
             break;
         }
         if (v != -1) {
+            // the jump recomputes every intersect, so the step's update is skipped
             const float toJump = -__int_as_float(v);
             trav += toJump;
-            cur += toJump;
-            sx = dx * cur + sx;
-            sy = dy * cur + sy;
-            sz = dz * cur + sz;
+            const float c2 = cur + toJump;
+            sx = dx * c2 + sx;
+            sy = dy * c2 + sy;
+            sz = dz * c2 + sz;
             cx = f2i(sx);
             cy = f2i(sy);
             cz = f2i(sz);
             div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok,
                     ix, iy, iz);
             idx = IX::make(g, cx, cy, cz);
+        } else {
+            ix = bx ? ix + ddx : ix;
+            iy = by ? iy + ddy : iy;
+            iz = bz ? iz + ddz : iz;
         }
     }
     h.index = result;
