@@ -6,7 +6,9 @@ frame per step, inputs resident in HBM.
 
 Prints one JSON line.  Besides the headline value it reports the dominant
 kernel's roofline (algorithmic bytes per launch / its average duration from
-HIP events on the launch stream, against 8 TB/s), the CPU oracle timed on
+HIP events on the launch stream, against 8 TB/s; and its VALU lane-operations
+per launch, from the committed counter summary, against the nominal VALU
+rate), the CPU oracle timed on
 this host as a baseline, and, unless --no-extra, the batched form (64 views of
 mixed resolution in one persistent launch vs 64 plain launches) and the
 served form (p50/p99 latency under a Poisson request stream with edits).
@@ -23,6 +25,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# 256 CUs x 64 lanes per clock (a wave64 VALU instruction every 4 clocks per
+# SIMD) x 2.4 GHz: the nominal non-packed VALU rate
+VALU_PEAK_T = 256 * 64 * 2.4e9 / 1e12
 METRIC = "Mrays/s (primary+shadow) at 1080p, 16 lights; p99 frame latency batched vs plain"
 
 
@@ -39,13 +44,15 @@ def load():
     return vrt
 
 
-def profiled_traffic():
-    """HBM bytes per k_render_plain launch measured with rocprofv3 --pmc
-    (calibrated FETCH_SIZE + WRITE_SIZE), from the committed counter summary."""
+def profiled(key):
+    """A per-launch figure of k_render_plain measured with rocprofv3 --pmc, from
+    the committed counter summary: "traffic_bytes_per_launch" (calibrated
+    FETCH_SIZE + WRITE_SIZE) or "valu_lane_ops_per_launch"
+    (SQ_THREAD_CYCLES_VALU, one per active lane per VALU instruction)."""
     path = os.path.join(ROOT, "profiles", "k_render_plain_c2_1080p_counters.md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -104,13 +111,19 @@ def plain_bench(vrt, steps, warmup):
     value = rays_per_frame * steps / (total_ms * 1e-3) / 1e6
     bytes_per_launch = 4 * steps_per_frame + pixels * (3 + 4)
     achieved = bytes_per_launch / (kernel_ms * 1e-3) / 1e9
+    lane_ops = profiled("valu_lane_ops_per_launch")
+    valu = None
+    if lane_ops:
+        valu_achieved = lane_ops / (kernel_ms * 1e-3) / 1e12
+        valu = {"achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "T lane-ops/s",
+                "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops}
     out = {
         "value": value, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kernel_ms,
         "kernel_ms_p50": float(np.percentile(per, 50)), "kernel_ms_p99": float(np.percentile(per, 99)),
         "rays_per_frame": int(rays_per_frame), "dda_steps_per_frame": int(steps_per_frame),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(),
-                     "bytes_per_launch": int(bytes_per_launch), "kernel": "k_render_plain"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled("traffic_bytes_per_launch"),
+                     "bytes_per_launch": int(bytes_per_launch), "kernel": "k_render_plain", "valu": valu},
     }
     return sc, ctx, out
 
