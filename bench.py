@@ -10,8 +10,9 @@ HIP events on the launch stream, against 8 TB/s; and its VALU lane-operations
 per launch, from the committed counter summary, against the nominal VALU
 rate), the CPU oracle timed on
 this host as a baseline, and, unless --no-extra, the batched form (64 views of
-mixed resolution in one persistent launch vs 64 plain launches) and the
-served form (p50/p99 latency under a Poisson request stream with edits).
+mixed resolution in one persistent launch vs 64 plain launches), the
+served form (p50/p99 latency under a Poisson request stream with edits) and
+the large world (8 GiB grid, 4K frames, shaded and depth view).
 """
 import argparse
 import json
@@ -282,6 +283,49 @@ def served_bench(vrt, n_requests=240, load=0.7):
     }
 
 
+def large_bench(vrt, frames=10):
+    """The large world (2048x512x2048, 8 GiB, 2^31 voxels, ~35% carved out
+    below the surface) generated on the device, its depth field, and 4K
+    frames from a grazing camera with a 4096-step cap, shaded and in the
+    depth-field view (the same scene the large-world parity test checks)."""
+    sc = vrt.make_scene("c3", seed=3)
+    ctx = vrt.Context(sc.dims)
+    t0 = time.perf_counter()
+    ctx.generate(sc.params)
+    ctx.sync()
+    gen_ms = (time.perf_counter() - t0) * 1e3
+    del sc.world
+    t0 = time.perf_counter()
+    ctx.build_depth_field()
+    ctx.sync()
+    df_ms = (time.perf_counter() - t0) * 1e3
+    tgt = vrt.Target(sc.width, sc.height, depth=True, aux=False)
+    ts = tgt.struct()
+    out = {"workload": "2048x512x2048 grid (8 GiB) generated on the device, 3840x2160, sun + 16 lights, "
+                       "render distance 4096, grazing camera",
+           "generate_ms": gen_ms, "depth_field_ms": df_ms}
+    for mode, key in ((0, "shaded"), (1, "depth_view")):
+        sc.frame.view_depth_field = mode
+        ctx.draw(ts, frame=sc.frame)
+        ctx.sync()
+        ctx.stats(reset=True)
+        evs = [vrt.Event() for _ in range(frames + 1)]
+        evs[0].record(ctx.stream)
+        for k in range(frames):
+            ctx.draw(ts, frame=sc.frame)
+            evs[k + 1].record(ctx.stream)
+        ctx.sync()
+        per = [evs[k + 1].ms_since(evs[k]) for k in range(frames)]
+        st = ctx.stats(reset=True)
+        ms = float(np.mean(per))
+        out[key] = {"kernel_ms": ms, "rays_per_frame": st["rays"] // frames,
+                    "dda_steps_per_frame": st["dda_steps"] // frames,
+                    "mrays_s": st["rays"] / (float(np.sum(per)) * 1e-3) / 1e6}
+    ctx.close()
+    del ts, tgt
+    return out
+
+
 def cpu_baseline(vrt, sc, ctx):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_ctypes as orc  # the oracle, timed as the CPU baseline only
@@ -306,7 +350,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--only", choices=["batched", "served"], default=None,
+    ap.add_argument("--only", choices=["batched", "served", "large"], default=None,
                     help="run only this leg (for profiling); prints its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -319,6 +363,9 @@ def main():
         return
     if args.only == "served":
         print(json.dumps(served_bench(vrt)), flush=True)
+        return
+    if args.only == "large":
+        print(json.dumps(large_bench(vrt)), flush=True)
         return
     sc, ctx, plain = plain_bench(vrt, args.steps, args.warmup)
     log("plain: %.1f Mrays/s, %.3f ms/frame" % (plain["value"], plain["kernel_ms"]))
@@ -346,6 +393,10 @@ def main():
         res["served"] = served_bench(vrt)
         log("served: p50 %.2f ms p99 %.2f ms, %.0f Mrays/s" % (
             res["served"]["latency_ms"]["p50"], res["served"]["latency_ms"]["p99"], res["served"]["mrays_s"]))
+        res["large"] = large_bench(vrt)
+        log("large: shaded %.2f ms (%.0f Mrays/s), depth view %.2f ms, depth field %.0f ms" % (
+            res["large"]["shaded"]["kernel_ms"], res["large"]["shaded"]["mrays_s"],
+            res["large"]["depth_view"]["kernel_ms"], res["large"]["depth_field_ms"]))
     print(json.dumps(res), flush=True)
 
 
