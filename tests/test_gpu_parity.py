@@ -168,6 +168,49 @@ def test_c2_frame_matches_oracle(gpu, orc):
     ctx.close()
 
 
+def entering_rays(dims, n_per_face, rng):
+    """Rays whose start cell is one cell outside the grid on one axis (-1 or
+    the dimension) and whose direction points into the grid on that axis."""
+    starts, dirs = [], []
+    for axis in range(3):
+        for low in (True, False):
+            st = np.stack([rng.uniform(0, dims[k], n_per_face) for k in range(3)], 1)
+            st[:, axis] = rng.uniform(-1.99, -1.0, n_per_face) if low else rng.uniform(dims[axis], dims[axis] + 0.99,
+                                                                                      n_per_face)
+            d = rng.normal(size=(n_per_face, 3)) * 0.6
+            d[:, axis] = np.abs(rng.normal(1.0, 0.3, n_per_face)) * (1.0 if low else -1.0)
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            starts.append(st)
+            dirs.append(d)
+    return np.concatenate(starts).astype(np.float32), np.concatenate(dirs).astype(np.float32)
+
+
+@pytest.mark.parametrize("dims", [(100, 37, 53), (4100, 2050, 2)], ids=["index32", "index64"])
+def test_cast_rays_entering_from_one_cell_outside(gpu, orc, dims):
+    """A start cell at -1 (or at the dimension) on an axis, stepping into the
+    grid: the voxel index of the entered cell must be exact although the start
+    cell's coordinates are out of range.  X and X*Y are not multiples of 256,
+    so a 24-bit unsigned multiply of -1 would give a wrong index (and a read
+    outside the grid); the second size takes the 64-bit index path."""
+    rng = np.random.default_rng(23)
+    X, Y, Z = dims
+    world = np.where(rng.random(X * Y * Z) < 0.08, rng.integers(0, 1 << 24, X * Y * Z), -1).astype(np.int32)
+    ctx = gpu.Context(dims)
+    ctx.upload(world)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    start, d = entering_rays(dims, 1500, rng)
+    dist = np.full(start.shape[0], 384, np.int32)
+    g = ctx.cast_rays(start, d, dist, 384)
+    r = orc.cast_rays(wdf, dims, start, d, dist, 384)
+    for k in ("hit", "face", "steps"):
+        bad = np.nonzero(g[k] != r[k])[0]
+        assert bad.size == 0, "%s differs for %d rays, first %d" % (k, bad.size, bad[0])
+    assert same_floats(g["hit_pos"], r["hit_pos"])
+    assert int((g["hit"] >= 0).sum()) > 1000
+    ctx.close()
+
+
 def test_cast_rays_extreme_values_match_oracle(gpu, orc):
     """int(float) at NaN, +-inf and beyond int32 range, zero and denormal
     direction components, huge budgets: the conversions saturate and NaN goes

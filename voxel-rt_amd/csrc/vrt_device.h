@@ -100,12 +100,17 @@ struct Hit {
     uint32_t iters;  // loop iterations (stepCount increments)
 };
 
-// Voxel index arithmetic: 32-bit with 24-bit multiplies when every index and
-// partial product fits (X*Y*Z <= 2^32, X and X*Y < 2^24), else 64-bit.
+// Voxel index arithmetic: 32-bit with signed 24-bit multiplies when every
+// index and partial product fits (X*Y*Z <= 2^32, X and X*Y < 2^23), else
+// 64-bit.  A start or jump cell may lie outside the grid, and one of -1 on an
+// axis can step into it, so make() must be exact modulo 2^32 for negative
+// coordinates too: the signed multiplies are, for |y|, |z| < 2^23 (cells
+// farther out cannot reach the grid in one step, and their index is never
+// read).
 struct Idx32 {
     using T = uint32_t;
     __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
-        return (uint32_t)x + __umul24((uint32_t)y, g.X) + __umul24((uint32_t)z, (uint32_t)g.XY);
+        return (uint32_t)x + (uint32_t)__mul24(y, (int32_t)g.X) + (uint32_t)__mul24(z, (int32_t)g.XY);
     }
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
         return axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.X : (uint32_t)sz * (uint32_t)g.XY);

@@ -28,7 +28,7 @@ struct GridDesc {
     uint64_t XY;
     uint64_t count;
     // Index fits the 32-bit / 24-bit-multiply form (see Idx32).
-    bool small() const { return count <= 0xffffffffull && X < (1u << 24) && XY < (1ull << 24); }
+    bool small() const { return count <= 0xffffffffull && X < (1u << 23) && XY < (1ull << 23); }
 };
 
 // One frame as the kernels see it: vrt_frame plus derived per-frame values.
