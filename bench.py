@@ -118,6 +118,22 @@ def plain_bench(vrt, steps, warmup):
         valu_achieved = lane_ops / (kernel_ms * 1e-3) / 1e12
         valu = {"achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "T lane-ops/s",
                 "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops}
+    # The same frames with the shadow cutoff off: every output is identical,
+    # the DDA walks are the reference's full length (its iteration count).
+    ctx.set_shadow_cutoff(False)
+    ctx.draw(ts, frame=sc.frame)
+    ctx.sync()
+    ctx.stats(reset=True)
+    evs[0].record(stream)
+    for k in range(steps):
+        ctx.draw(ts, frame=sc.frame)
+        evs[k + 1].record(stream)
+    ctx.sync()
+    per_full = [evs[k + 1].ms_since(evs[k]) for k in range(steps)]
+    st_full = ctx.stats(reset=True)
+    ctx.set_shadow_cutoff(True)
+    full = {"value": st_full["rays"] / (float(np.sum(per_full)) * 1e-3) / 1e6,
+            "kernel_ms": float(np.mean(per_full)), "dda_steps_per_frame": int(st_full["dda_steps"] // steps)}
     out = {
         "value": value, "ms_per_step": wall * 1e3 / steps, "kernel_ms": kernel_ms,
         "kernel_ms_p50": float(np.percentile(per, 50)), "kernel_ms_p99": float(np.percentile(per, 99)),
@@ -125,6 +141,7 @@ def plain_bench(vrt, steps, warmup):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled("traffic_bytes_per_launch"),
                      "bytes_per_launch": int(bytes_per_launch), "kernel": "k_render_plain", "valu": valu},
+        "shadow_cutoff_off": full,
     }
     return sc, ctx, out
 
@@ -368,7 +385,9 @@ def main():
         print(json.dumps(large_bench(vrt)), flush=True)
         return
     sc, ctx, plain = plain_bench(vrt, args.steps, args.warmup)
-    log("plain: %.1f Mrays/s, %.3f ms/frame" % (plain["value"], plain["kernel_ms"]))
+    log("plain: %.1f Mrays/s, %.3f ms/frame (shadow cutoff off: %.1f Mrays/s, %.3f ms)" % (
+        plain["value"], plain["kernel_ms"], plain["shadow_cutoff_off"]["value"],
+        plain["shadow_cutoff_off"]["kernel_ms"]))
     res = {
         "metric": METRIC, "value": plain["value"], "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": plain["ms_per_step"], "higher_is_better": True,
@@ -380,6 +399,7 @@ def main():
         "kernel_ms": plain["kernel_ms"], "kernel_ms_p50": plain["kernel_ms_p50"],
         "kernel_ms_p99": plain["kernel_ms_p99"], "rays_per_frame": plain["rays_per_frame"],
         "dda_steps_per_frame": plain["dda_steps_per_frame"], "roofline": plain["roofline"],
+        "shadow_cutoff_off": plain["shadow_cutoff_off"],
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -95,7 +95,8 @@ typedef struct vrt_target {
 /* Counters accumulated by every draw of a context. */
 typedef struct vrt_stats {
     uint64_t rays;       /* primary + shadow rays cast */
-    uint64_t dda_steps;  /* DDA loop iterations over all rays (one 4-byte voxel read each) */
+    uint64_t dda_steps;  /* DDA loop iterations executed over all rays (one 4-byte voxel read
+                          * each); equal to the reference's count with the shadow cutoff off */
     uint64_t pixels;
     uint64_t draws;
 } vrt_stats;
@@ -116,8 +117,21 @@ int vrt_destroy(vrt_ctx* ctx);
 int vrt_set_stream(vrt_ctx* ctx, void* stream);
 void* vrt_get_stream(vrt_ctx* ctx);
 int vrt_sync(vrt_ctx* ctx);
-/* Device pointer of the grid (int32_t[X*Y*Z]). */
+/* Device pointer of the grid (int32_t[X*Y*Z]).  After writing solid voxels
+ * through it, call vrt_grid_modified before the next draw. */
 int vrt_grid_device_ptr(vrt_ctx* ctx, int32_t** out);
+/* Re-derives what the context keeps about the grid's contents (the highest y
+ * of any solid voxel, used by the shadow cutoff) after writes made through
+ * vrt_grid_device_ptr.  The library's own uploads and edits keep it current.
+ * No counterpart in the reference, whose grid has no device-side writers. */
+int vrt_grid_modified(vrt_ctx* ctx);
+/* Shadow cutoff (on by default): a shadow ray that is above every solid voxel
+ * of the grid and not heading down stops there as a miss instead of walking on
+ * to the grid's edge or its budget.  Every output equals the reference's
+ * either way; only vrt_stats.dda_steps (iterations executed) differs: with the
+ * cutoff off it equals the reference's count.  Applies to later draws and to
+ * batches created later.  No counterpart in the reference. */
+int vrt_set_shadow_cutoff(vrt_ctx* ctx, int32_t enable);
 int vrt_grid_dims(vrt_ctx* ctx, int64_t dims[3]);
 
 /* ---- uploads: updateGeometry / updatePartialGeometry -------------------- */

@@ -93,7 +93,11 @@ def test_batch_views_match_oracle(gpu, orc, scene):
     arr = view_array(gpu, views, targets)
     ctx.stats(reset=True)
     b = C.c_void_p()
-    gpu.call("vrt_batch_create", ctx.h, arr, len(views), None, C.byref(b))
+    ctx.set_shadow_cutoff(False)  # so that the DDA-step total is the reference's
+    try:
+        gpu.call("vrt_batch_create", ctx.h, arr, len(views), None, C.byref(b))
+    finally:
+        ctx.set_shadow_cutoff(True)
     gpu.call("vrt_batch_launch", b)
     st = ctx.stats(reset=True)
     rays = steps = 0

@@ -1,8 +1,9 @@
 """GPU parity: libvrt's HIP kernels against the CPU oracle, bit for bit.
 
 Bar: hit index, face, shadow-evaluated and shadow-occluded bits, primary step
-counts, ray and DDA-step totals and depth-field words bit-exact; RGB8 within
-1 LSB per channel (tolerance written in assert_frames_match)."""
+counts, ray totals and depth-field words bit-exact; RGB8 within 1 LSB per
+channel (tolerance written in assert_frames_match); DDA-step totals equal to
+the oracle's with the shadow cutoff off, and never above them with it on."""
 import numpy as np
 import pytest
 
@@ -29,7 +30,12 @@ def assert_frames_match(g, r, label=""):
     d = np.abs(g["rgb"].astype(np.int16) - r["rgb"].astype(np.int16))
     assert int(d.max()) <= RGB_TOL, "%s rgb differs by %d" % (label, int(d.max()))
     assert g["rays"] == r["rays"], "%s rays %d vs %d" % (label, g["rays"], r["rays"])
-    assert g["dda_steps"] == r["dda_steps"], "%s steps %d vs %d" % (label, g["dda_steps"], r["dda_steps"])
+    # the shadow cutoff stops shadow rays early, never late; without it the
+    # iteration totals are the reference's
+    if g.get("shadow_cutoff", True):
+        assert g["dda_steps"] <= r["dda_steps"], "%s steps %d > %d" % (label, g["dda_steps"], r["dda_steps"])
+    else:
+        assert g["dda_steps"] == r["dda_steps"], "%s steps %d vs %d" % (label, g["dda_steps"], r["dda_steps"])
 
 
 @pytest.mark.parametrize("name", ["tiny", "c1"])
@@ -54,6 +60,72 @@ def test_frame_matches_oracle(gpu, orc, name, seed, mode):
     r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height)
     assert_frames_match(g, r, "%s/%d/%d" % (name, seed, mode))
     assert (r["hit"] >= 0).mean() > 0.2  # the frame sees terrain
+
+
+@pytest.mark.parametrize("name,seed", [("tiny", 1), ("c1", 4)])
+def test_shadow_cutoff_changes_only_the_step_total(gpu, orc, name, seed):
+    """Every output is the same with the shadow cutoff on and off; off, the
+    DDA-step total is the oracle's; on, it is smaller (sun rays rise above
+    the terrain and stop)."""
+    sc = gpu.make_scene(name, seed=seed)
+    wdf = orc.depth_field(sc.world, sc.dims)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    on = gpu.render_on_gpu(sc, aux=True, ctx=ctx, shadow_cutoff=True)
+    off = gpu.render_on_gpu(sc, aux=True, ctx=ctx, shadow_cutoff=False)
+    r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height)
+    assert_frames_match(off, r, "cutoff off")
+    assert_frames_match(on, r, "cutoff on")
+    for k in ("rgb", "depth", "hit", "face", "shadow_eval", "shadow_occ", "steps"):
+        assert on[k].tobytes() == off[k].tobytes(), k
+    assert on["dda_steps"] < off["dda_steps"] == r["dda_steps"]
+    ctx.close()
+
+
+@pytest.mark.parametrize("how", ["upload_region", "place_voxel", "device_pointer"])
+def test_shadow_cutoff_follows_new_solid_voxels(gpu, orc, how):
+    """A roof of solid voxels added 10 cells above the terrain's top must shade
+    what lies under it: sun rays jump through the gap above the old top, so a
+    solid-top bound that did not rise with the new voxels would end them there
+    as misses.  One case per way of adding voxels."""
+    sc = gpu.make_scene("tiny", seed=1)
+    X, Y, Z = sc.dims
+    world = sc.world.copy()
+    old_top = int(np.nonzero((world.reshape(Z, Y, X) >= 0).any(axis=(0, 2)))[0].max())
+    ry = min(Y - 2, old_top + 10)
+    assert sc.frame.cam_pos[1] < ry
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(world)
+    roof = world.reshape(Z, Y, X).copy()
+    # x in [0, X), z in [0, Z-1): what one vrt_upload_region call with an
+    # in-grid end corner can cover (the reference's updatePartialGeometry rule)
+    roof[:Z - 1, ry, :] = 0x808080
+    roof = roof.reshape(-1)
+    if how == "upload_region":
+        ctx.upload_region(roof, (0, ry, 0), (X - 1, ry + 1, Z - 1))
+    elif how == "place_voxel":
+        for z in range(Z - 1):
+            for x in range(X):
+                ctx.place_voxel(x, ry, z, 0x808080)
+    else:
+        p = gpu.P()
+        gpu.call("vrt_grid_device_ptr", ctx.h, gpu.C.byref(p))
+        buf = gpu.DeviceBuffer(4 * roof.size)
+        buf.upload(roof)
+        gpu.hip_check(gpu.hip().hipMemcpy(p, buf.ptr, buf.nbytes, gpu.D2D), "hipMemcpy D2D")
+        gpu.hip_check(gpu.hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+        ctx.grid_modified()
+    ctx.build_depth_field()
+    g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
+    assert np.array_equal(ctx.download() >= 0, roof >= 0)
+    r = orc.render(orc.depth_field(roof, sc.dims), sc.dims, sc.frame, sc.width, sc.height)
+    assert_frames_match(g, r, "roof via " + how)
+    # the roof shades many pixels that the open sky left lit
+    r0 = orc.render(orc.depth_field(world, sc.dims), sc.dims, sc.frame, sc.width, sc.height)
+    newly = (r["shadow_occ"] & 1) & ~(r0["shadow_occ"] & 1)
+    assert int(newly.sum()) > 1000
+    ctx.close()
 
 
 def test_cast_rays_match_oracle(gpu, orc):

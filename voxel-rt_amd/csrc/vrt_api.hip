@@ -147,6 +147,7 @@ void release(vrt_ctx* c) {
     hipSetDevice(c->device);
     if (c->own_stream) hipStreamSynchronize(c->own_stream);
     if (c->grid.vox) hipFree(c->grid.vox);
+    if (c->grid.top) hipFree(c->grid.top);
     if (c->counters) hipFree(c->counters);
     if (c->table_packed) hipFree(c->table_packed);
     if (c->table_dist) hipFree(c->table_dist);
@@ -194,6 +195,11 @@ void mat4_mul(const float* a, const float* b, float* out) {
 const float kIdentity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
 
 }  // namespace
+
+int solid_top_full(vrt_ctx* c) {
+    return launch_solid_top(c->grid, 0, 0, 0, c->grid.X, c->grid.Y, c->grid.Z, true, c->stream);
+}
+
 }  // namespace vrt
 
 using namespace vrt;
@@ -247,6 +253,7 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
     c->grid.Z = (uint32_t)Z;
     c->grid.XY = (uint64_t)X * (uint64_t)Y;
     c->grid.count = count;
+    if ((e = hipMalloc(&c->grid.top, sizeof(int32_t))) != hipSuccess) return bail(hip_fail(e, "hipMalloc top"));
     const size_t counter_bytes = (size_t)kCounterSlots * kCounterStride * sizeof(unsigned long long);
     if ((e = hipMalloc(&c->counters, counter_bytes)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc counters"));
@@ -257,6 +264,8 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
     if ((e = hipMemsetAsync(c->counters, 0, counter_bytes, c->stream)) != hipSuccess)
         return bail(hip_fail(e, "hipMemsetAsync"));
     rc = launch_fill(c->grid.vox, count, -1, c->stream);
+    if (rc) return bail(rc);
+    rc = launch_solid_top(c->grid, 0, 0, 0, 0, 0, 0, true, c->stream);  // no solid voxel yet: -1
     if (rc) return bail(rc);
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(hip_fail(e, "hipStreamSynchronize"));
     vrt_frame_defaults(&c->frame, X, 800, 600);
@@ -305,6 +314,7 @@ int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels) {
     if (!host_voxels) return fail(VRT_ERR_INVALID, "null voxels");
     VRT_HIP(hipMemcpyAsync(ctx->grid.vox, host_voxels, ctx->grid.count * sizeof(int32_t), hipMemcpyHostToDevice,
                            ctx->stream));
+    if (int rc = solid_top_full(ctx)) return rc;
     VRT_HIP(hipStreamSynchronize(ctx->stream));
     return VRT_OK;
 }
@@ -312,7 +322,19 @@ int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels) {
 int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels) {
     if (int rc = check_ctx(ctx)) return rc;
     if (!device_voxels) return fail(VRT_ERR_INVALID, "null voxels");
-    return launch_copy(ctx->grid.vox, device_voxels, ctx->grid.count, ctx->stream);
+    if (int rc = launch_copy(ctx->grid.vox, device_voxels, ctx->grid.count, ctx->stream)) return rc;
+    return solid_top_full(ctx);
+}
+
+int vrt_grid_modified(vrt_ctx* ctx) {
+    if (int rc = check_ctx(ctx)) return rc;
+    return solid_top_full(ctx);
+}
+
+int vrt_set_shadow_cutoff(vrt_ctx* ctx, int32_t enable) {
+    if (int rc = check_ctx(ctx)) return rc;
+    ctx->shadow_cutoff = enable != 0;
+    return VRT_OK;
 }
 
 // updatePartialGeometry, src/render.cpp:204-223: corners swapped when the
@@ -348,6 +370,8 @@ int vrt_upload_region(vrt_ctx* ctx, const int32_t* host_voxels, int32_t x0, int3
     p.extent = make_hipExtent(row_bytes, ye - ys, ze - zs);
     p.kind = hipMemcpyHostToDevice;
     VRT_HIP(hipMemcpy3DAsync(&p, ctx->stream));
+    // the region can only have gained solid voxels above the old bound
+    if (int rc = launch_solid_top(g, xs, ys, zs, xe + 1, ye, ze, false, ctx->stream)) return rc;
     VRT_HIP(hipStreamSynchronize(ctx->stream));
     return VRT_OK;
 }
@@ -408,7 +432,7 @@ int vrt_place_voxel(vrt_ctx* ctx, int32_t x, int32_t y, int32_t z, int32_t voxel
     if (int rc = check_ctx(ctx)) return rc;
     const GridDesc& g = ctx->grid;
     if (x < 0 || y < 0 || z < 0 || (uint32_t)x >= g.X || (uint32_t)y >= g.Y || (uint32_t)z >= g.Z) return VRT_OK;
-    return launch_set_voxel(g, (int64_t)x + (int64_t)g.X * y + (int64_t)g.XY * z, voxel, ctx->stream);
+    return launch_set_voxel(g, (int64_t)x + (int64_t)g.X * y + (int64_t)g.XY * z, y, voxel, ctx->stream);
 }
 
 // destroyVoxel, src/render.cpp:265-271.
@@ -514,7 +538,7 @@ int vrt_draw_frame(vrt_ctx* ctx, const vrt_frame* f, const vrt_target* t) {
     if (int rc = check_frame(*f)) return rc;
     if (int rc = check_target(*t)) return rc;
     ctx->draws++;
-    return launch_render_plain(ctx->grid, make_frame_desc(*f), *t, ctx->counters, ctx->stream);
+    return launch_render_plain(render_grid(ctx), make_frame_desc(*f), *t, ctx->counters, ctx->stream);
 }
 
 // glDrawArrays over the fullscreen quad, src/main.cpp:59.
@@ -577,7 +601,8 @@ int vrt_selftest_division(vrt_ctx* ctx, uint32_t d_begin, uint32_t d_count, uint
 int vrt_world_generate_device(vrt_ctx* ctx, const vrt_world_params* p) {
     if (int rc = check_ctx(ctx)) return rc;
     if (!p) return fail(VRT_ERR_INVALID, "null params");
-    return world_generate_device(ctx->grid, *p, ctx->stream);
+    if (int rc = world_generate_device(ctx->grid, *p, ctx->stream)) return rc;
+    return solid_top_full(ctx);
 }
 
 }  // extern "C"

@@ -135,7 +135,7 @@ int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_b
         return bail(hip_fail(e, "hipStreamBeginCapture"));
     int rc = launch_queue_reset(b->queue, b->capture);
     if (!rc)
-        rc = launch_render_batch(ctx->grid, b->views_dev, (int32_t)b->views.size(), b->total_tiles, cfg, b->workgroups,
+        rc = launch_render_batch(render_grid(ctx), b->views_dev, (int32_t)b->views.size(), b->total_tiles, cfg, b->workgroups,
                                  b->queue, ctx->counters, b->capture);
     e = hipStreamEndCapture(b->capture, &b->graph);
     if (rc) return bail(rc);

@@ -133,9 +133,24 @@ struct Idx64 {
 // is only used while the cell is inside the grid (per-axis bounds, :42-45):
 // between two in-grid cells every step was checked, so it never wrapped.
 // The axis choice is computed with selects so the wave does not branch.
+//
+// ytop (shadow rays): an upper bound on the y of every solid voxel, or
+// INT32_MAX.  Let dy >= 0.  Every cell of a segment has y >= int(start.y),
+// and a jump sets start.y += RN(dy * c) with c = the entry intersect plus the
+// jump.  A negative entry intersect either belongs to an axis whose step
+// leaves the grid (the walk ends before any jump), or comes from a zero
+// direction component, which then makes c infinite or NaN at every jump from
+// the first one on.  So after a jump that leaves int(start.y) > ytop with c
+// finite and >= 0, every later jump keeps c >= 0, start.y never decreases,
+// and no cell the walk can still reach is solid: the ray is a miss.  The loop
+// stops there instead of walking on to the grid's edge or its budget; the
+// result is the reference's, only the iteration count is smaller.  (A jump
+// with c = -inf or NaN leaves int(start.y) at INT32_MIN or 0, where the test
+// fires only if the grid has no solid voxel; c = +inf makes `travelled`
+// infinite, which ends the walk at the next loop test anyway.)
 template <typename IX>
 __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, float sz, float dx, float dy,
-                                         float dz, float lim, Hit& h) {
+                                         float dz, float lim, Hit& h, int32_t ytop = INT32_MAX) {
     using T = typename IX::T;
     int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
     const int32_t stx = isign(dx), sty = isign(dy), stz = isign(dz);
@@ -192,6 +207,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             cx = f2i(sx);
             cy = f2i(sy);
             cz = f2i(sz);
+            if (cy > ytop) break;
             div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok,
                     ix, iy, iz);
             idx = IX::make(g, cx, cy, cz);
@@ -220,9 +236,10 @@ struct PixelResult {
 };
 
 // main(), src/fshader.glsl:131-190, for the fragment at vPos = (vx, vy).
+// `top`: the grid's solid-top bound for shadow rays (INT32_MAX: none).
 template <typename IX>
 __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& F, float vx, float vy,
-                                            PixelResult& o) {
+                                            PixelResult& o, int32_t top) {
     const vrt_frame& f = F.f;
     float col0 = 0.6f, col1 = 0.7f, col2 = 0.8f;
 
@@ -274,7 +291,8 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
         // sun shadow (:152-156)
         {
             Hit s;
-            cast_ray<IX>(g, hx + tlx * 0.001f, hy + tly * 0.001f, hz + tlz * 0.001f, tlx, tly, tlz, F.render_lim, s);
+            cast_ray<IX>(g, hx + tlx * 0.001f, hy + tly * 0.001f, hz + tlz * 0.001f, tlx, tly, tlz, F.render_lim, s,
+                         tly >= 0.0f ? top : INT32_MAX);
             iters += s.iters;
             rays++;
             eval |= 1u;
@@ -308,7 +326,8 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             float lim = (float)f2i(dist + 1.0f);
             lim = lim < F.render_lim ? lim : F.render_lim;
             Hit s;
-            cast_ray<IX>(g, hx + ux1 * 0.001f, hy + uy1 * 0.001f, hz + uz1 * 0.001f, ux1, uy1, uz1, lim, s);
+            cast_ray<IX>(g, hx + ux1 * 0.001f, hy + uy1 * 0.001f, hz + uz1 * 0.001f, ux1, uy1, uz1, lim, s,
+                         uy1 >= 0.0f ? top : INT32_MAX);
             iters += s.iters;
             rays++;
             eval |= 2u << i;

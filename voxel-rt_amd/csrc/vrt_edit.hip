@@ -108,8 +108,35 @@ __global__ void __launch_bounds__(256) k_carve(GridDesc g, int32_t x0, int32_t y
     g.vox[(uint64_t)x + g.X * (uint64_t)y + g.XY * (uint64_t)z] = -1;
 }
 
-__global__ void k_set_voxel(int32_t* vox, uint64_t i, int32_t v) {
-    if (threadIdx.x == 0) vox[i] = v;
+__global__ void k_set_voxel(int32_t* vox, uint64_t i, int32_t v, int32_t* top, int32_t y) {
+    if (threadIdx.x == 0) {
+        vox[i] = v;
+        if (v >= 0 && top) atomicMax(top, y);
+    }
+}
+
+__global__ void k_set_word(int32_t* p, int32_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+// Highest y with a solid voxel over rows (y, z) of the box [x0, x0+w) x
+// [y0, y0+h) x [z0, z0+d): a wave reads a row's words with its 64 lanes, keeps
+// the highest y of its rows with any solid word and raises *top once.
+__global__ void __launch_bounds__(256) k_solid_top(GridDesc g, int32_t x0, int32_t w, int32_t y0, int32_t h,
+                                                    int32_t z0, int64_t rows, int32_t* top) {
+    const int32_t lane = threadIdx.x & 63;
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    int32_t best = -1;
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < rows; r += waves) {
+        const int32_t y = y0 + (int32_t)(r % h);
+        const int32_t z = z0 + (int32_t)(r / h);
+        if (y <= best) continue;
+        const int32_t* row = g.vox + (uint64_t)g.X * (uint64_t)y + g.XY * (uint64_t)z + (uint64_t)x0;
+        bool any = false;
+        for (int32_t x = lane; x < w; x += 64) any |= row[x] >= 0;
+        if (__builtin_amdgcn_ballot_w64(any) != 0) best = y;
+    }
+    if (lane == 0 && best >= 0) atomicMax(top, best);
 }
 
 // Grid copy, 16 bytes per lane per step (a plain streaming copy of known size).
@@ -182,9 +209,29 @@ int launch_carve_sphere(const GridDesc& g, int32_t cx, int32_t cy, int32_t cz, i
     return VRT_OK;
 }
 
-int launch_set_voxel(const GridDesc& g, int64_t index, int32_t v, hipStream_t s) {
+int launch_set_voxel(const GridDesc& g, int64_t index, int32_t y, int32_t v, hipStream_t s) {
     if (index < 0 || (uint64_t)index >= g.count) return fail(VRT_ERR_INVALID, "voxel index out of range");
-    hipLaunchKernelGGL(k_set_voxel, dim3(1), dim3(64), 0, s, g.vox, (uint64_t)index, v);
+    hipLaunchKernelGGL(k_set_voxel, dim3(1), dim3(64), 0, s, g.vox, (uint64_t)index, v, g.top, y);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_solid_top(const GridDesc& g, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1, int64_t z1,
+                     bool reset, hipStream_t s) {
+    if (!g.top) return fail(VRT_ERR_INVALID, "grid without a solid-top word");
+    if (reset) {
+        hipLaunchKernelGGL(k_set_word, dim3(1), dim3(64), 0, s, g.top, -1);
+        VRT_HIP(hipGetLastError());
+    }
+    clip(x0, x1, g.X);
+    clip(y0, y1, g.Y);
+    clip(z0, z1, g.Z);
+    if (x1 <= x0 || y1 <= y0 || z1 <= z0) return VRT_OK;
+    const int64_t rows = (y1 - y0) * (z1 - z0);
+    int64_t blocks = (rows + 3) / 4;  // 4 waves per block, one row each to start
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_solid_top, dim3((uint32_t)blocks), dim3(256), 0, s, g, (int32_t)x0, (int32_t)(x1 - x0),
+                       (int32_t)y0, (int32_t)(y1 - y0), (int32_t)z0, rows, g.top);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }

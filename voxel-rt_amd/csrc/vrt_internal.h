@@ -27,6 +27,12 @@ struct GridDesc {
     uint32_t X, Y, Z;
     uint64_t XY;
     uint64_t count;
+    // Device word: an upper bound on the y of every solid voxel (-1 when there
+    // is none).  Every entry point that can add solid voxels raises it; edits
+    // that only remove leave it high, which stays an upper bound.  Render
+    // kernels stop a shadow ray once it is provably above it; null: they walk
+    // every shadow ray to the reference's end.
+    int32_t* top;
     // Index fits the 32-bit / 24-bit-multiply form (see Idx32).
     bool small() const { return count <= 0xffffffffull && X < (1u << 23) && XY < (1ull << 23); }
 };
@@ -85,7 +91,11 @@ int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, 
                          int32_t x1, int32_t y1, int32_t z1, int32_t sphere_cx, int32_t sphere_cy,
                          int32_t sphere_cz, int32_t sphere_r, uint8_t* brick_scratch, hipStream_t s);
 int launch_carve_sphere(const GridDesc& g, int32_t cx, int32_t cy, int32_t cz, int32_t r, hipStream_t s);
-int launch_set_voxel(const GridDesc& g, int64_t index, int32_t v, hipStream_t s);
+int launch_set_voxel(const GridDesc& g, int64_t index, int32_t y, int32_t v, hipStream_t s);
+// Raises *g.top to the highest y with a solid voxel in the box [x0,x1) x
+// [y0,y1) x [z0,z1) (clipped to the grid); with reset, sets it to -1 first.
+int launch_solid_top(const GridDesc& g, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1, int64_t z1,
+                     bool reset, hipStream_t s);
 int launch_fill(int32_t* p, uint64_t n, int32_t v, hipStream_t s);
 int launch_copy(int32_t* dst, const int32_t* src, uint64_t n, hipStream_t s);
 int launch_world_generate(const GridDesc& g, const vrt_world_params& p, const int32_t* cave_list,
@@ -125,7 +135,18 @@ struct vrt_ctx {
     int32_t num_cus = 0;
     bool tuned = false;
     vrt_batch_config tuned_cfg{8, 8, 4, 0};
+    bool shadow_cutoff = true;  // vrt_set_shadow_cutoff
 };
+
+namespace vrt {
+// The grid as the render kernels get it: `top` only with the shadow cutoff on.
+inline GridDesc render_grid(const vrt_ctx* c) {
+    GridDesc g = c->grid;
+    if (!c->shadow_cutoff) g.top = nullptr;
+    return g;
+}
+int solid_top_full(vrt_ctx* c);  // recomputes *grid.top over the whole grid
+}  // namespace vrt
 
 namespace vrt {
 int world_generate_device(const GridDesc& g, const vrt_world_params& p, hipStream_t s);

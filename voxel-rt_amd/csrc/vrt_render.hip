@@ -52,6 +52,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
 template <typename IX>
 __global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counters) {
+    const int32_t top = g.top ? *g.top : INT32_MAX;
     const uint32_t lb = blockIdx.x;
     const int32_t tiles_x = (t.width + 7) >> 3;
     const int32_t tiles_y = (t.height + 7) >> 3;
@@ -65,7 +66,7 @@ k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counte
             float vx, vy;
             dev::pixel_vpos(t.width, t.height, px, row, vx, vy);
             dev::PixelResult p;
-            dev::shade_pixel<IX>(g, F, vx, vy, p);
+            dev::shade_pixel<IX>(g, F, vx, vy, p, top);
             dev::store_pixel(t, (int64_t)row * t.width + px, p);
             rays = p.rays;
             iters = p.iters;
@@ -110,6 +111,7 @@ template <typename IX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t tw,
                int32_t th, int32_t chunk, unsigned int* queue, unsigned long long* counters) {
+    const int32_t top = g.top ? *g.top : INT32_MAX;
     const int32_t lane = threadIdx.x & 63;
     const int32_t rows_per_pass = kWave / tw;
     const int32_t passes = th / rows_per_pass;
@@ -148,7 +150,7 @@ k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, 
                         float vx, vy;
                         dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
                         dev::PixelResult p;
-                        dev::shade_pixel<IX>(g, V.frame, vx, vy, p);
+                        dev::shade_pixel<IX>(g, V.frame, vx, vy, p, top);
                         dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
                         rays += p.rays;
                         iters += p.iters;

@@ -95,6 +95,8 @@ SIGNATURES = {
     "vrt_sync": (C.c_int, [P]),
     "vrt_grid_device_ptr": (C.c_int, [P, C.POINTER(P)]),
     "vrt_grid_dims": (C.c_int, [P, C.POINTER(I64)]),
+    "vrt_grid_modified": (C.c_int, [P]),
+    "vrt_set_shadow_cutoff": (C.c_int, [P, I32]),
     "vrt_upload": (C.c_int, [P, P]),
     "vrt_upload_device": (C.c_int, [P, P]),
     "vrt_upload_region": (C.c_int, [P, P, I32, I32, I32, I32, I32, I32]),
@@ -356,6 +358,16 @@ class Context:
         self.dims = tuple(int(d) for d in dims)
         self.h = P()
         call("vrt_create", C.byref(self.h), self.dims[0], self.dims[1], self.dims[2], device)
+        self.shadow_cutoff = True
+
+    def set_shadow_cutoff(self, enable):
+        """Shadow rays above every solid voxel stop early (outputs unchanged;
+        with it off the DDA-step totals equal the reference's)."""
+        call("vrt_set_shadow_cutoff", self.h, 1 if enable else 0)
+        self.shadow_cutoff = bool(enable)
+
+    def grid_modified(self):
+        call("vrt_grid_modified", self.h)
 
     def close(self):
         if self.h:
@@ -628,20 +640,23 @@ def make_views(params, dims, world, seed, n_views, resolutions=MIXED_RESOLUTIONS
     return views
 
 
-def render_on_gpu(scene, aux=True, depth=True, ctx=None):
+def render_on_gpu(scene, aux=True, depth=True, ctx=None, shadow_cutoff=True):
     """Uploads the scene's world, builds its depth field on the device and draws
-    one frame; returns the host copies of the outputs plus ray/step counts."""
+    one frame; returns the host copies of the outputs plus ray/step counts
+    (and whether the shadow cutoff was on)."""
     own = ctx is None
     if own:
         ctx = Context(scene.dims)
         ctx.upload(scene.world)
         ctx.build_depth_field()
+    ctx.set_shadow_cutoff(shadow_cutoff)
     tgt = Target(scene.width, scene.height, depth=depth, aux=aux)
     ctx.stats(reset=True)
     ctx.draw(tgt, frame=scene.frame)
     st = ctx.stats(reset=True)
     out = tgt.download()
     out.update(st)
+    out["shadow_cutoff"] = bool(shadow_cutoff)
     if own:
         ctx.close()
     return out
