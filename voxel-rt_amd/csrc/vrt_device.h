@@ -212,6 +212,23 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
                     ix, iy, iz);
             idx = IX::make(g, cx, cy, cz);
         } else {
+            // The axis just taken has step 0 (its direction component is +-0,
+            // so its intersect is +-inf or NaN and adding its delta keeps it
+            // so): the cell did not move, and every later iteration repeats
+            // this one exactly -- same selection, same cell, same -1 word --
+            // until the budget ends the walk as a miss.  Only the counters
+            // advance, so they are advanced without the loads.  (The
+            // reference's own light orbit keeps lightPos.z = X/2: a sun ray
+            // from a hit on the plane z = X/2 has dir.z = +0 and a NaN z
+            // intersect, so z is always the axis taken, and it sticks here
+            // when its cell holds -1.)
+            if ((bx ? stx : (by ? sty : stz)) == 0) {
+                while (trav < lim) {
+                    iters++;
+                    trav += 1.0f;
+                }
+                break;
+            }
             ix = bx ? ix + ddx : ix;
             iy = by ? iy + ddy : iy;
             iz = bz ? iz + ddz : iz;
