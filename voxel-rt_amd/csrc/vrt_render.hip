@@ -1,14 +1,16 @@
 // vrt_render.hip — render kernels for gfx950 (wave64).
 //
-//   k_render_plain : one frame; a one-wave workgroup shades one 8x8 screen
-//                    tile (one ray per lane), tiles in row-major order.  The
-//                    hardware deals workgroups to the 8 XCDs round-robin, so
-//                    every XCD gets an even share of sky, horizon and terrain
-//                    tiles; giving each XCD a contiguous band of rows instead
-//                    (for L2 locality) made the XCDs' loads unequal and the
-//                    frame 29% slower.  One wave per workgroup lets a wave
-//                    slot refill as soon as its tile is done (with four, the
-//                    slot waited for the slowest sibling; 2% slower).
+//   k_render_plain : one frame; a wave shades one 8x8 screen tile (one ray per
+//                    lane), a 256-thread workgroup four horizontally adjacent
+//                    tiles, in row-major order.  The four share one CU and its
+//                    L1: on an 8 GiB grid, one-wave workgroups (tiles spread
+//                    over CUs) made 4K frames 38% slower, 2x2 or 4x2 tile
+//                    groups 3-5% slower; on the 256 MiB grid all of these are
+//                    within 2%.  The hardware deals workgroups to the 8 XCDs
+//                    round-robin, so every XCD gets an even share of sky,
+//                    horizon and terrain tiles; giving each XCD a contiguous
+//                    band of rows instead (for L2 locality) made the XCDs'
+//                    loads unequal and the frame 29% slower.
 //   k_render_batch : many views in one persistent launch; every wave pops
 //                    chunks of (view, tile) work from a device atomic queue
 //                    until the queue is empty (the exit every wave reaches).
@@ -26,7 +28,7 @@ namespace vrt {
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kPlainWaves = 1;                 // waves per workgroup
+constexpr int kPlainWaves = 4;                 // waves per workgroup
 constexpr int kPlainThreads = kWave * kPlainWaves;
 constexpr int kXcds = 8;
 constexpr int kQueueStride = 32;  // queue counters 128 bytes apart
