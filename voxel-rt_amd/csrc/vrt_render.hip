@@ -4,13 +4,14 @@
 //                    lane), a 256-thread workgroup four horizontally adjacent
 //                    tiles, in row-major order.  The four share one CU and its
 //                    L1: on an 8 GiB grid, one-wave workgroups (tiles spread
-//                    over CUs) made 4K frames 38% slower, 2x2 or 4x2 tile
-//                    groups 3-5% slower; on the 256 MiB grid all of these are
-//                    within 2%.  The hardware deals workgroups to the 8 XCDs
-//                    round-robin, so every XCD gets an even share of sky,
-//                    horizon and terrain tiles; giving each XCD a contiguous
-//                    band of rows instead (for L2 locality) made the XCDs'
-//                    loads unequal and the frame 29% slower.
+//                    over CUs) made 4K frames 38% slower, 2x2, 4x2 or 8x1 tile
+//                    groups 3-4% slower (on the 256 MiB grid one-wave groups
+//                    were 2% faster, 2x2 even).  The hardware deals
+//                    workgroups to the 8 XCDs round-robin, so every XCD gets
+//                    an even share of sky, horizon and terrain tiles; giving
+//                    each XCD a contiguous band of rows instead (for L2
+//                    locality) made the XCDs' loads unequal and the frame 29%
+//                    slower.
 //   k_render_batch : many views in one persistent launch; every wave pops
 //                    chunks of (view, tile) work from a device atomic queue
 //                    until the queue is empty (the exit every wave reaches).
