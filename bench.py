@@ -177,7 +177,8 @@ def percentiles(xs):
 def batched_bench(vrt, reps=5):
     """64 views of resolution uniform in {320x180 .. 3840x2160}, random poses
     over a 1024x256x1024 world, 0-16 lights each: 64 plain launches vs one
-    persistent launch (autotuned tile shape and queue chunk, hipGraph)."""
+    batch launch (autotuned tile shape and scheduling -- hardware-scheduled
+    workgroups or persistent waves with a given queue chunk -- as a hipGraph)."""
     C = vrt.C
     params, dims, world, ctx = world_ctx(vrt, "c4", 4)
     views = vrt.make_views(params, dims, world, 44, 64)
@@ -206,7 +207,7 @@ def batched_bench(vrt, reps=5):
     st = ctx.stats(reset=True)
     rays = st["rays"] // reps
     steps = st["dda_steps"] // reps
-    # autotune, then the tuned persistent launch as a graph
+    # autotune, then the tuned launch as a graph
     best = vrt.vrt_batch_config()
     res = (vrt.vrt_tune_entry * 32)()
     n = C.c_int32(0)

@@ -216,7 +216,7 @@ int vrt_cast_rays(vrt_ctx* ctx, int64_t n, const float* start, const float* dir,
                   const int32_t* dist, int32_t render_dist,
                   int64_t* hit, float* hit_pos, uint8_t* face, uint32_t* steps);
 
-/* ---- batched form: many views in one persistent launch ------------------ */
+/* ---- batched form: many views in one launch ------------------------------ */
 
 typedef struct vrt_view {
     vrt_frame frame;
@@ -224,9 +224,11 @@ typedef struct vrt_view {
 } vrt_view;
 
 typedef struct vrt_batch_config {
-    int32_t tile_w;        /* screen tile per queue item: 8x8, 16x4, 32x2 or 16x16 */
+    int32_t tile_w;        /* screen tile per work item: 8x8, 16x4, 32x2 or 16x16 */
     int32_t tile_h;
-    int32_t chunk_tiles;   /* tiles taken per queue pop */
+    int32_t chunk_tiles;   /* 0: the hardware schedules workgroups of four tiles (no
+                            * queue); >= 1: persistent waves pop this many tiles per
+                            * device-queue pop */
     int32_t waves_per_cu;  /* persistent waves per CU (0 = from occupancy) */
 } vrt_batch_config;
 
@@ -237,10 +239,10 @@ typedef struct vrt_tune_entry {
 
 typedef struct vrt_batch vrt_batch;
 
-/* A persistent-kernel plan over a fixed set of views: descriptors are
- * uploaded once, the queue reset plus kernel are captured in a hipGraph.
- * cfg NULL = the tuned config for this context if vrt_autotune_batch ran,
- * else 8x8 tiles, 4 tiles per pop. */
+/* A plan over a fixed set of views: descriptors are uploaded once, the
+ * launch (with the queue reset when chunk_tiles >= 1) is captured in a
+ * hipGraph.  cfg NULL = the tuned config for this context if
+ * vrt_autotune_batch ran, else 8x8 tiles, hardware-scheduled. */
 int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n,
                      const vrt_batch_config* cfg, vrt_batch** out);
 int vrt_batch_launch(vrt_batch* b);
@@ -248,7 +250,7 @@ int vrt_batch_destroy(vrt_batch* b);
 /* Renders each view with its own plain launch, one after another. */
 int vrt_draw_views_plain(vrt_ctx* ctx, const vrt_view* views, int32_t n);
 /* Times every tile shape in {8x8,16x4,32x2,16x16} x chunk size in
- * {1,2,4,8,16} on these views and keeps the fastest as the context's
+ * {0,1,2,4,8,16} on these views and keeps the fastest as the context's
  * default.  results (may be NULL) receives up to max_results entries. */
 int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, int32_t n, int32_t reps,
                        vrt_batch_config* best, vrt_tune_entry* results,
@@ -258,7 +260,7 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, int32_t n, int32_t r
 
 typedef struct vrt_server_config {
     uint64_t framebuffer_budget;   /* bytes of frame buffers admitted at once */
-    int32_t max_batch_views;       /* views per persistent launch */
+    int32_t max_batch_views;       /* views per batch launch */
     int32_t streams;               /* batches that may run on the device together */
     int32_t want_depth;            /* allocate and fill depth per request */
     int32_t want_aux;              /* allocate and fill hit/face/shadow/steps maps */

@@ -61,7 +61,7 @@ def plain_outputs(gpu, ctx, views):
 
 
 @pytest.mark.parametrize("tw,th", [(8, 8), (16, 4), (32, 2), (16, 16)])
-@pytest.mark.parametrize("chunk", [1, 3, 16])
+@pytest.mark.parametrize("chunk", [0, 1, 3, 16])
 def test_batch_equals_plain(gpu, scene, tw, th, chunk):
     sc, ctx, _ = scene
     views = small_views(gpu, sc, 12, seed=tw * 100 + chunk)
@@ -121,11 +121,11 @@ def test_autotune_picks_a_valid_config(gpu, scene):
     res = (gpu.vrt_tune_entry * 32)()
     n = C.c_int32(0)
     gpu.call("vrt_autotune_batch", ctx.h, arr, len(views), 2, C.byref(best), res, 32, C.byref(n))
-    assert n.value == 20
+    assert n.value == 24  # 4 tile shapes x chunk 0 (hardware-scheduled), 1, 2, 4, 8, 16
     shapes = {(res[i].config.tile_w, res[i].config.tile_h) for i in range(n.value)}
     assert shapes == {(8, 8), (16, 4), (32, 2), (16, 16)}
     assert min(res[i].ms for i in range(n.value)) == min(r.ms for r in res[:n.value])
-    assert (best.tile_w, best.tile_h) in shapes and best.chunk_tiles in (1, 2, 4, 8, 16)
+    assert (best.tile_w, best.tile_h) in shapes and best.chunk_tiles in (0, 1, 2, 4, 8, 16)
 
 
 # ------------------------------------------------------------------ server

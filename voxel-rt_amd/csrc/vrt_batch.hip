@@ -1,10 +1,11 @@
 // vrt_batch.hip — batched form: many camera views of uneven resolution in one
-// persistent launch.  A plan uploads the views' descriptors once (largest
-// view first, so the long-running tiles start early and small views fill the
-// tail), then captures "reset the tile queue; launch the persistent kernel"
-// into a hipGraph that every vrt_batch_launch replays.  The queue reset is a
-// one-thread kernel, not a memset node.  The autotuner times every tile shape
-// and queue chunk size and keeps the fastest for the context.
+// launch.  A plan uploads the views' descriptors once (largest view first, so
+// the long-running tiles start early and small views fill the tail), then
+// captures the launch into a hipGraph that every vrt_batch_launch replays:
+// either the hardware-scheduled kernel (chunk_tiles 0), or "reset the tile
+// queue; launch the persistent kernel" (the reset is a one-thread kernel, not
+// a memset node).  The autotuner times every tile shape and chunk size and
+// keeps the fastest for the context.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,7 +37,7 @@ int check_config(const vrt_batch_config& c) {
                           (c.tile_w == 32 && c.tile_h == 2) || (c.tile_w == 16 && c.tile_h == 16) ||
                           (c.tile_w == 64 && c.tile_h == 1);
     if (!shape_ok) return fail(VRT_ERR_INVALID, "tile shape must be 8x8, 16x4, 32x2, 16x16 or 64x1");
-    if (c.chunk_tiles < 1 || c.chunk_tiles > 4096) return fail(VRT_ERR_INVALID, "chunk_tiles must be in [1, 4096]");
+    if (c.chunk_tiles < 0 || c.chunk_tiles > 4096) return fail(VRT_ERR_INVALID, "chunk_tiles must be in [0, 4096]");
     if (c.waves_per_cu < 0 || c.waves_per_cu > 64) return fail(VRT_ERR_INVALID, "waves_per_cu must be in [0, 64]");
     return VRT_OK;
 }
@@ -87,6 +88,10 @@ int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& c
 }
 
 int persistent_workgroups(vrt_ctx* c, const vrt_batch_config& cfg, int64_t total_tiles, int32_t* out) {
+    if (cfg.chunk_tiles == 0) {  // hardware-scheduled: one workgroup per four tiles, no queue
+        *out = (int32_t)std::min<int64_t>((total_tiles + 3) / 4, 0x7fffffff);
+        return VRT_OK;
+    }
     int32_t waves = cfg.waves_per_cu;
     if (waves == 0) {
         if (int rc = batch_occupancy(cfg, &waves)) return rc;
@@ -133,7 +138,7 @@ int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_b
         return bail(hip_fail(e, "hipStreamCreate"));
     if ((e = hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess)
         return bail(hip_fail(e, "hipStreamBeginCapture"));
-    int rc = launch_queue_reset(b->queue, b->capture);
+    int rc = cfg.chunk_tiles ? launch_queue_reset(b->queue, b->capture) : VRT_OK;
     if (!rc)
         rc = launch_render_batch(render_grid(ctx), b->views_dev, (int32_t)b->views.size(), b->total_tiles, cfg, b->workgroups,
                                  b->queue, ctx->counters, b->capture);
@@ -173,7 +178,7 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, int32_t n, int32_t r
     if (reps < 1) reps = 1;
     VRT_HIP(hipSetDevice(ctx->device));
     const int32_t shapes[4][2] = {{8, 8}, {16, 4}, {32, 2}, {16, 16}};
-    const int32_t chunks[5] = {1, 2, 4, 8, 16};
+    const int32_t chunks[6] = {0, 1, 2, 4, 8, 16};  // 0: hardware-scheduled, no queue
     std::vector<vrt_tune_entry> all;
     hipEvent_t e0, e1;
     VRT_HIP(hipEventCreate(&e0));

@@ -134,7 +134,7 @@ struct vrt_ctx {
     uint8_t* bricks = nullptr;
     int32_t num_cus = 0;
     bool tuned = false;
-    vrt_batch_config tuned_cfg{8, 8, 4, 0};
+    vrt_batch_config tuned_cfg{8, 8, 0, 0};
     bool shadow_cutoff = true;  // vrt_set_shadow_cutoff
 };
 

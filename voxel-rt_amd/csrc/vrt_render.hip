@@ -12,6 +12,8 @@
 //                    each XCD a contiguous band of rows instead (for L2
 //                    locality) made the XCDs' loads unequal and the frame 29%
 //                    slower.
+//   k_render_views : many views in one launch, four consecutive (view, tile)
+//                    items per workgroup, scheduled by the hardware.
 //   k_render_batch : many views in one persistent launch; every wave pops
 //                    chunks of (view, tile) work from a device atomic queue
 //                    until the queue is empty (the exit every wave reaches).
@@ -103,26 +105,77 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
     if (steps) steps[i] = h.iters;
 }
 
-// Persistent multi-view kernel.  Tile shape tw x th with tw*th a multiple of
-// 64: a wave shades 64 pixels per pass (lane -> (lane % tw, lane / tw)) and
-// tw*th/64 passes per tile.  The tiles (views largest first, each row-major)
-// are split into kXcds contiguous ranges with one queue counter each, on its
-// own cache line; a wave drains the range of the XCD it runs on (workgroups
-// are dispatched round-robin over the XCDs, so blockIdx.x mod 8) and then
-// steals from the others.  A pop takes `chunk` tiles; every wave leaves once
-// all ranges are empty.
+// One tile of a multi-view launch: `tile` indexes the concatenated tile lists
+// of the views (largest view first, each row-major).  Tile shape tw x th with
+// tw*th a multiple of 64: the wave shades 64 pixels per pass (lane ->
+// (lane % tw, lane / tw)) and tw*th/64 passes.  Per-lane totals stay far
+// below 2^32 for any launch the host accepts (tiles per launch < 2^32; 17
+// rays and at most 17 * 2^24 steps a pixel would overflow only past ~15
+// pixels at the step cap); they are widened when the wave flushes.
+template <typename IX>
+__device__ __forceinline__ void shade_view_tile(const GridDesc& g, const ViewDesc* __restrict__ views,
+                                                int32_t n_views, int64_t tile, int32_t tw, int32_t th,
+                                                int32_t lane, int32_t top, uint32_t& rays, uint32_t& iters,
+                                                uint32_t& pixels) {
+    const int32_t rows_per_pass = kWave / tw;
+    const int32_t passes = th / rows_per_pass;
+    // view owning this tile (uniform across the wave)
+    int32_t lo = 0, hi = n_views - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (views[mid].first_tile <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    const ViewDesc& V = views[__builtin_amdgcn_readfirstlane(lo)];
+    const int32_t local = (int32_t)(tile - V.first_tile);
+    const int32_t tx0 = (local % V.tiles_x) * tw;
+    const int32_t ty0 = (local / V.tiles_x) * th;
+    for (int32_t pass = 0; pass < passes; pass++) {
+        const int32_t px = tx0 + lane % tw;
+        const int32_t row = ty0 + pass * rows_per_pass + lane / tw;
+        if (px < V.target.width && row < V.target.height) {
+            float vx, vy;
+            dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
+            dev::PixelResult p;
+            dev::shade_pixel<IX>(g, V.frame, vx, vy, p, top);
+            dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
+            rays += p.rays;
+            iters += p.iters;
+            pixels += 1;
+        }
+    }
+}
+
+// Multi-view launch scheduled by the hardware: a 256-thread workgroup takes
+// four consecutive tiles (one per wave, sharing the CU's L1), and the
+// dispatcher hands out workgroups in order as slots free up, so the views'
+// tiles are balanced over the device without any queue.
+template <typename IX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k_render_views(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t tw,
+               int32_t th, unsigned long long* counters) {
+    const int32_t top = g.top ? *g.top : INT32_MAX;
+    const int32_t lane = threadIdx.x & 63;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t rays = 0, iters = 0, pixels = 0;
+    if (tile < total_tiles) shade_view_tile<IX>(g, views, n_views, tile, tw, th, lane, top, rays, iters, pixels);
+    flush_counters(counters, rays, iters, pixels);
+}
+
+// Persistent multi-view kernel.  The tiles are split into kXcds contiguous
+// ranges with one queue counter each, on its own cache line; a wave drains
+// the range of the XCD it runs on (workgroups are dispatched round-robin over
+// the XCDs, so blockIdx.x mod 8) and then steals from the others.  A pop takes
+// `chunk` tiles; every wave leaves once all ranges are empty.  Pops are
+// device-scope atomics on one word per XCD, so they pay off only when tiles
+// are large: on one 1080p frame of 8x8 tiles this kernel ran 50% slower than
+// k_render_views (2.8x slower with a single counter for all XCDs).
 template <typename IX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t tw,
                int32_t th, int32_t chunk, unsigned int* queue, unsigned long long* counters) {
     const int32_t top = g.top ? *g.top : INT32_MAX;
     const int32_t lane = threadIdx.x & 63;
-    const int32_t rows_per_pass = kWave / tw;
-    const int32_t passes = th / rows_per_pass;
-    // per-lane totals stay far below 2^32 for any launch the host accepts
-    // (tiles per launch < 2^32, 17 rays and at most 17 * 2^24 steps a pixel
-    // would overflow only past ~15 pixels at the step cap; counts are widened
-    // when the wave flushes)
     uint32_t rays = 0, iters = 0, pixels = 0;
     const uint32_t home = blockIdx.x % kXcds;
     for (uint32_t k = 0; k < kXcds; k++) {
@@ -135,33 +188,8 @@ k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, 
             if ((int64_t)base >= r1 - r0) break;
             int64_t end = r0 + (int64_t)base + chunk;
             if (end > r1) end = r1;
-            for (int64_t tile = r0 + base; tile < end; tile++) {
-                // view owning this tile (uniform across the wave)
-                int32_t lo = 0, hi = n_views - 1;
-                while (lo < hi) {
-                    const int32_t mid = (lo + hi + 1) >> 1;
-                    if (views[mid].first_tile <= tile) lo = mid;
-                    else hi = mid - 1;
-                }
-                const ViewDesc& V = views[__builtin_amdgcn_readfirstlane(lo)];
-                const int32_t local = (int32_t)(tile - V.first_tile);
-                const int32_t tx0 = (local % V.tiles_x) * tw;
-                const int32_t ty0 = (local / V.tiles_x) * th;
-                for (int32_t pass = 0; pass < passes; pass++) {
-                    const int32_t px = tx0 + lane % tw;
-                    const int32_t row = ty0 + pass * rows_per_pass + lane / tw;
-                    if (px < V.target.width && row < V.target.height) {
-                        float vx, vy;
-                        dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
-                        dev::PixelResult p;
-                        dev::shade_pixel<IX>(g, V.frame, vx, vy, p, top);
-                        dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
-                        rays += p.rays;
-                        iters += p.iters;
-                        pixels += 1;
-                    }
-                }
-            }
+            for (int64_t tile = r0 + base; tile < end; tile++)
+                shade_view_tile<IX>(g, views, n_views, tile, tw, th, lane, top, rays, iters, pixels);
         }
     }
     flush_counters(counters, rays, iters, pixels);
@@ -234,6 +262,19 @@ int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu) {
 int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_views, int64_t total_tiles,
                         const vrt_batch_config& cfg, int32_t n_workgroups, unsigned int* queue,
                         unsigned long long* counters, hipStream_t s) {
+    if (cfg.chunk_tiles == 0) {
+        const int64_t wgs = (total_tiles + 3) / 4;
+        if (wgs <= 0) return VRT_OK;
+        if (wgs > 0x7fffffff) return fail(VRT_ERR_INVALID, "too many tiles in one batch");
+        if (g.small())
+            hipLaunchKernelGGL(k_render_views<dev::Idx32>, dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev,
+                               n_views, total_tiles, cfg.tile_w, cfg.tile_h, counters);
+        else
+            hipLaunchKernelGGL(k_render_views<dev::Idx64>, dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev,
+                               n_views, total_tiles, cfg.tile_w, cfg.tile_h, counters);
+        VRT_HIP(hipGetLastError());
+        return VRT_OK;
+    }
     if (n_workgroups <= 0) return fail(VRT_ERR_INVALID, "no workgroups");
     if (g.small())
         hipLaunchKernelGGL(k_render_batch<dev::Idx32>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,

@@ -197,7 +197,8 @@ int launch_slot(vrt_server* s, Slot& sl, std::vector<uint64_t>& ids) {
     if (s->have_edit) VRT_HIP(hipStreamWaitEvent(sl.stream, s->edit_done, 0));
     VRT_HIP(hipMemcpyAsync(sl.views_dev, sl.views_host, descs.size() * sizeof(ViewDesc), hipMemcpyHostToDevice,
                            sl.stream));
-    if (int rc = launch_queue_reset(sl.queue, sl.stream)) return rc;
+    if (cfg.chunk_tiles)
+        if (int rc = launch_queue_reset(sl.queue, sl.stream)) return rc;
     if (int rc = launch_render_batch(render_grid(c), sl.views_dev, (int32_t)descs.size(), total, cfg, wgs, sl.queue,
                                      c->counters, sl.stream))
         return rc;
