@@ -34,6 +34,11 @@
  *     overbright early-out fires when all three channels are >= 1.25, which
  *     is the scalar rule when r = g = b;
  *   - float32 depth = length(hitPos - camPos), +inf on a miss.
+ *
+ * ORC_MUTANT (default 0): building with -DORC_MUTANT=k (k >= 1) plants one
+ * deliberate, plausible mistake (the MUT(k) sites below).  Only
+ * tests/test_oracle_mutants.py builds those variants, to show that the pins
+ * in tests/test_oracle_pins.py fail on each of them.
  */
 #include <cmath>
 #include <cstdint>
@@ -43,6 +48,11 @@
 #include <omp.h>
 
 #include "../include/vrt.h"
+
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT(k) (ORC_MUTANT == (k))
 
 namespace {
 
@@ -91,11 +101,17 @@ inline void normalize3(const float v[3], float out[3]) {
     out[0] = v[0] * inv;
     out[1] = v[1] * inv;
     out[2] = v[2] * inv;
+    if (MUT(6)) {  /* v / length(v) instead of v * (1 / length(v)) */
+        float len = std::sqrt(dot3(v, v));
+        out[0] = v[0] / len;
+        out[1] = v[1] / len;
+        out[2] = v[2] / len;
+    }
 }
 
 inline uint8_t unorm8(float c) {
     float k = c > 0.0f ? (c < 1.0f ? c : 1.0f) : 0.0f;
-    return (uint8_t)(int32_t)(k * 255.0f + 0.5f);
+    return (uint8_t)(int32_t)(k * 255.0f + (MUT(10) ? 0.0f : 0.5f));
 }
 
 /* Globals of the shader that castRay writes (src/fshader.glsl:28-31). */
@@ -113,16 +129,21 @@ int64_t cast_ray(const Grid& g, const float startIn[3], const float rayDirection
                  int32_t dist, int32_t renderDist, ShaderGlobals& G) {
     float startPosition[3] = {startIn[0], startIn[1], startIn[2]};
     int32_t currCheck[3] = {f2i(startPosition[0]), f2i(startPosition[1]), f2i(startPosition[2])};
+    if (MUT(3))  /* floor instead of truncation */
+        for (int a = 0; a < 3; a++) currCheck[a] = f2i(std::floor(startPosition[a]));
     int64_t fColorIndex = -1;
     int64_t tempIndex = -1;
 
     int32_t step[3] = {f2i(glsl_sign(rayDirection[0])), f2i(glsl_sign(rayDirection[1])),
                        f2i(glsl_sign(rayDirection[2]))};
     int32_t forwardSteps[3] = {(int32_t)(step[0] > 0), (int32_t)(step[1] > 0), (int32_t)(step[2] > 0)};
+    if (MUT(5))  /* step >= 0 counted as forward */
+        for (int a = 0; a < 3; a++) forwardSteps[a] = (int32_t)(step[a] >= 0);
 
-    float dx = 1.0f / std::fabs(rayDirection[0] + 0.000001f);
-    float dy = 1.0f / std::fabs(rayDirection[1] + 0.000001f);
-    float dz = 1.0f / std::fabs(rayDirection[2] + 0.000001f);
+    const float bias = MUT(2) ? -0.000001f : 0.000001f;  /* MUT(2): sign of the 1e-6 flipped */
+    float dx = 1.0f / std::fabs(rayDirection[0] + bias);
+    float dy = 1.0f / std::fabs(rayDirection[1] + bias);
+    float dz = 1.0f / std::fabs(rayDirection[2] + bias);
 
     float intersect[3];
     for (int a = 0; a < 3; a++)
@@ -134,7 +155,9 @@ int64_t cast_ray(const Grid& g, const float startIn[3], const float rayDirection
         G.stepCount += 1.0f;
         distTravelled += 1.0f;
         G.loopIters++;
-        if (intersect[0] < intersect[1] && intersect[0] < intersect[2]) {
+        const bool takeX = MUT(1) ? (intersect[0] <= intersect[1] && intersect[0] <= intersect[2])  /* ties to x */
+                                  : (intersect[0] < intersect[1] && intersect[0] < intersect[2]);
+        if (takeX) {
             currDist = intersect[0];
             currCheck[0] += step[0];
             intersect[0] += dx;
@@ -162,9 +185,10 @@ int64_t cast_ray(const Grid& g, const float startIn[3], const float rayDirection
         } else if (tempIndex >= 0 && g.vox[tempIndex] != -1) {
             float toJump = -int_bits_to_float(g.vox[tempIndex]);
             distTravelled += toJump;
-            currDist += toJump;
+            if (!MUT(4)) currDist += toJump;  /* MUT(4): the jump does not move the start */
+            if (MUT(13)) G.stepCount += 1.0f;  /* MUT(13): a jump counted as two steps */
             for (int a = 0; a < 3; a++) startPosition[a] = rayDirection[a] * currDist + startPosition[a];
-            for (int a = 0; a < 3; a++) currCheck[a] = f2i(startPosition[a]);
+            for (int a = 0; a < 3; a++) currCheck[a] = f2i(MUT(3) ? std::floor(startPosition[a]) : startPosition[a]);
             for (int a = 0; a < 3; a++)
                 intersect[a] = ((float)(currCheck[a] + forwardSteps[a]) - startPosition[a]) / rayDirection[a];
         } else if (tempIndex < 0) {
@@ -185,19 +209,30 @@ struct PixelOut {
     uint64_t iters;
 };
 
+/* Primary ray of src/fshader.glsl:136-137 for vPos = (vx, vy):
+ * rotateMatrix * vec4(normalize(vec3(vPos.x * aspectRatio, vPos.y, focal)), 0)
+ * with the column-major matrix and GLM's pairwise sum order. */
+void primary_dir(const vrt_frame& f, float vx, float vy, float out[3]) {
+    float unrot[3] = {vx * f.aspect_ratio, vy, f.focal};
+    if (MUT(23)) { unrot[0] = vx; unrot[1] = vy * f.aspect_ratio; }  /* aspect on the wrong axis */
+    float d[3];
+    normalize3(unrot, d);
+    if (MUT(11)) { d[0] = unrot[0]; d[1] = unrot[1]; d[2] = unrot[2]; }  /* not normalized */
+    const float* M = f.rotate_matrix;
+    for (int c = 0; c < 3; c++)
+        out[c] = (M[0 + c] * d[0] + M[4 + c] * d[1]) + (M[8 + c] * d[2] + M[12 + c] * 0.0f);
+    if (MUT(12))  /* the matrix read row-major (transposed) */
+        for (int c = 0; c < 3; c++)
+            out[c] = (M[4 * c + 0] * d[0] + M[4 * c + 1] * d[1]) + (M[4 * c + 2] * d[2] + M[4 * c + 3] * 0.0f);
+}
+
 /* main(), src/fshader.glsl:131-190, for the fragment at vPos = (vx, vy). */
 void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut& o) {
     ShaderGlobals G;
     float fColor[3] = {0.6f, 0.7f, 0.8f};
 
-    float unrot[3] = {vx * f.aspect_ratio, vy, f.focal};
-    float rayDirection[3];
-    normalize3(unrot, rayDirection);
-    const float* M = f.rotate_matrix;
     float rotatedDir[3];
-    for (int c = 0; c < 3; c++)
-        rotatedDir[c] = (M[0 + c] * rayDirection[0] + M[4 + c] * rayDirection[1]) +
-                        (M[8 + c] * rayDirection[2] + M[12 + c] * 0.0f);
+    primary_dir(f, vx, vy, rotatedDir);
 
     int64_t fColorIndex = cast_ray(g, f.cam_pos, rotatedDir, f.render_dist, f.render_dist, G);
     float firstHitPos[3] = {G.hitPos[0], G.hitPos[1], G.hitPos[2]};
@@ -219,7 +254,7 @@ void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut
     }
 
     if (f.view_depth_field == 1) {
-        float s = G.stepCount / 100.0f;
+        float s = G.stepCount / (MUT(22) ? 255.0f : 100.0f);  /* MUT(22): wrong scale */
         fColor[0] = s; fColor[1] = s; fColor[2] = s;
     } else {
         float lv[3] = {f.light_pos[0] - firstHitPos[0], f.light_pos[1] - firstHitPos[1],
@@ -234,7 +269,7 @@ void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut
             o.rays++;
             o.eval |= 1u;
             if (cast_ray(g, so, toLight, f.render_dist, f.render_dist, G) == -1) {
-                float lam = max0(dot3(firstHitNormal, toLight));
+                float lam = MUT(17) ? dot3(firstHitNormal, toLight) : max0(dot3(firstHitNormal, toLight));
                 for (int c = 0; c < 3; c++) multiplier[c] += 0.8f * lam;   /* DIFFUSE */
             } else {
                 o.occ |= 1u;
@@ -242,23 +277,28 @@ void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut
 
             for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
                 const vrt_local_light& L = f.lights[i];
-                if (multiplier[0] >= 1.25f && multiplier[1] >= 1.25f && multiplier[2] >= 1.25f) {
+                const bool over = MUT(7) ? (multiplier[0] > 1.25f && multiplier[1] > 1.25f && multiplier[2] > 1.25f)
+                                         : (multiplier[0] >= 1.25f && multiplier[1] >= 1.25f && multiplier[2] >= 1.25f);
+                if (over) {
                     multiplier[0] = 1.25f; multiplier[1] = 1.25f; multiplier[2] = 1.25f;  /* MAX_OVERBRIGHT */
                     break;
-                } else if (L.pos[0] >= 0.0f && L.pos[1] >= 0.0f && L.pos[2] >= 0.0f) {
+                } else if (MUT(18) ? (L.pos[0] > 0.0f && L.pos[1] > 0.0f && L.pos[2] > 0.0f)
+                                   : (L.pos[0] >= 0.0f && L.pos[1] >= 0.0f && L.pos[2] >= 0.0f)) {
                     float lvec[3] = {L.pos[0] - firstHitPos[0], L.pos[1] - firstHitPos[1],
                                      L.pos[2] - firstHitPos[2]};
                     float localLightDist = length3(lvec);
-                    if (localLightDist <= L.radius) {
+                    if (MUT(8) ? localLightDist < L.radius : localLightDist <= L.radius) {
                         float toLocalLight[3];
                         normalize3(lvec, toLocalLight);
                         float lo[3];
                         for (int a = 0; a < 3; a++) lo[a] = firstHitPos[a] + toLocalLight[a] * 0.001f;
                         o.rays++;
                         o.eval |= 2u << i;
-                        if (cast_ray(g, lo, toLocalLight, f2i(localLightDist + 1.0f), f.render_dist, G) == -1) {
+                        const int32_t budget = MUT(20) ? f.render_dist : f2i(localLightDist + 1.0f);  /* MUT(20): no distance budget */
+                        if (cast_ray(g, lo, toLocalLight, budget, f.render_dist, G) == -1) {
                             float lam = max0(dot3(firstHitNormal, toLocalLight));
                             float fall = (L.radius - localLightDist) / L.radius;
+                            if (MUT(16)) fall = localLightDist / L.radius;  /* the "r -" term dropped */
                             for (int c = 0; c < 3; c++) multiplier[c] += L.rgb[c] * lam * fall;
                         } else {
                             o.occ |= 2u << i;
@@ -268,6 +308,7 @@ void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut
             }
 
             int32_t v = g.vox[fColorIndex];
+            if (MUT(19)) v = ((v & 0xFF) << 16) | (v & 0xFF00) | ((v >> 16) & 0xFF);  /* red and blue swapped */
             fColor[0] = (float)((v & 0x00FF0000) >> 16) / 255.0f * multiplier[0];
             fColor[1] = (float)((v & 0x0000FF00) >> 8) / 255.0f * multiplier[1];
             fColor[2] = (float)(v & 0x000000FF) / 255.0f * multiplier[2];
@@ -296,6 +337,11 @@ std::vector<DepthEntry> make_depth_table() {
                     int xDist = xCheck - (xCheck > 0) + (xCheck < 0);
                     int yDist = yCheck - (yCheck > 0) + (yCheck < 0);
                     int zDist = zCheck - (zCheck > 0) + (zCheck < 0);
+                    if (MUT(9)) {  /* offsets not shrunk toward 0 */
+                        xDist = xCheck;
+                        yDist = yCheck;
+                        zDist = zCheck;
+                    }
                     /* sqrt of an int is the double overload; the result is
                      * stored in the float `dist` (src/render.cpp:67,81). */
                     dist = (float)(-std::sqrt((double)(xDist * xDist + yDist * yDist + zDist * zDist)));
@@ -324,7 +370,7 @@ int32_t fix_depth_value(const Grid& g, int32_t x, int32_t y, int32_t z) {
             int64_t indexCheck = voxel_index(g, x + T[i].x, y + T[i].y, z + T[i].z);
             bool solid = indexCheck >= 0 && g.vox[indexCheck] >= 0;
             if (solid && dist > nearestDist) {
-                if (dist <= -2.0f) nearestDist = dist;
+                if (dist <= -2.0f || MUT(15)) nearestDist = dist;  /* MUT(15): -2 < d* stored too */
                 else nearestDist = 0.0f;
             }
         }
@@ -403,9 +449,9 @@ void orc_remove_sphere(int32_t* vox, int64_t X, int64_t Y, int64_t Z, int32_t px
         for (int y = -radius; y < radius; y++)
             for (int x = -radius; x < radius; x++)
                 if (x + px < X && y + py < Y && z + pz < Z && x + px >= 0 && y + py >= 0 && z + pz >= 0)
-                    if (x * x + y * y + z * z < radius * radius)
+                    if (MUT(14) ? x * x + y * y + z * z <= radius * radius : x * x + y * y + z * z < radius * radius)
                         orc_destroy_voxel(vox, X, Y, Z, x + px, y + py, z + pz);
-    radius += VRT_DEPTH_FIELD_RADIUS >> 1;
+    if (!MUT(21)) radius += VRT_DEPTH_FIELD_RADIUS >> 1;  /* MUT(21): repair only the carved ball */
     for (int z = -radius; z < radius; z++)
         for (int y = -radius; y < radius; y++)
             for (int x = -radius; x < radius; x++)
@@ -509,14 +555,7 @@ void orc_normalize(const float* v, float* out) { normalize3(v, out); }
 float orc_length(const float* v) { return length3(v); }
 float orc_dot(const float* a, const float* b) { return dot3(a, b); }
 /* Primary ray of src/fshader.glsl:136-137 for vPos = (vx, vy). */
-void orc_primary_dir(const vrt_frame* f, float vx, float vy, float* out) {
-    float unrot[3] = {vx * f->aspect_ratio, vy, f->focal};
-    float d[3];
-    normalize3(unrot, d);
-    const float* M = f->rotate_matrix;
-    for (int c = 0; c < 3; c++)
-        out[c] = (M[0 + c] * d[0] + M[4 + c] * d[1]) + (M[8 + c] * d[2] + M[12 + c] * 0.0f);
-}
+void orc_primary_dir(const vrt_frame* f, float vx, float vy, float* out) { primary_dir(*f, vx, vy, out); }
 }
 
 /* initVoxels, placeTrunk, placeBush: src/level.cpp:4-27,59-138, transcribed

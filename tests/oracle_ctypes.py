@@ -8,7 +8,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB_PATH = os.path.join(ORACLE_DIR, "liboracle.so")
+# VRT_ORACLE_LIB names another build of the oracle; only the mutant test sets it,
+# to run the pins against deliberately broken builds.
+LIB_PATH = os.environ.get("VRT_ORACLE_LIB") or os.path.join(ORACLE_DIR, "liboracle.so")
 GLM_PROBE = os.path.join(ORACLE_DIR, "_ref", "glm_probe")
 
 P = C.c_void_p
@@ -22,7 +24,7 @@ class orc_outputs(C.Structure):
 def lib():
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        if not os.path.exists(LIB_PATH) and not os.environ.get("VRT_ORACLE_LIB"):
             subprocess.run(["make", "-s", "-C", ORACLE_DIR, "liboracle.so"], check=True)
         L = C.CDLL(LIB_PATH)
         i64, i32 = C.c_int64, C.c_int32

@@ -374,3 +374,55 @@ def test_degenerate_worlds_and_ragged_frames(gpu, orc, idx):
         r = orc.render(wdf, dims, f, W, H)
         assert_frames_match(g, r, "%s %dx%d rd=%d" % (name, W, H, rd))
     ctx.close()
+
+
+def test_pin_rays_match_oracle(gpu, orc):
+    """The closed-form rays that pin the oracle (start-cell truncation, exactly
+    zero direction components, the 1e-6 step bias) through the kernels."""
+    import test_oracle_pins as pins
+    for make in (pins.truncation_rays, pins.zero_component_rays, pins.step_bias_rays):
+        dims, w, start, d = make()
+        start = np.asarray(start, np.float32)
+        d = np.asarray(d, np.float32)
+        dist = np.full(start.shape[0], 384, np.int32)
+        ctx = gpu.Context(dims)
+        ctx.upload(w)
+        g = ctx.cast_rays(start, d, dist, 384)
+        r = orc.cast_rays(w, dims, start, d, dist, 384)
+        for k in ("hit", "face", "steps"):
+            assert np.array_equal(g[k], r[k]), "%s: %s gpu %s oracle %s" % (make.__name__, k, g[k], r[k])
+        assert same_floats(g["hit_pos"], r["hit_pos"]), make.__name__
+        ctx.close()
+
+
+@pytest.mark.parametrize("case", ["range", "overbright", "falloff27", "falloff28", "roof"])
+def test_pin_frames_match_oracle(gpu, orc, case):
+    """The exact local-light scenes that pin the oracle (light at exactly its
+    radius, overbright reached at exactly 1.25, linear falloff, shadow rays
+    that stop at the light) through the kernels, cutoff off and on."""
+    import test_oracle_pins as pins
+    if case == "roof":
+        dims, w = pins.roof_world()
+        f = pins.roof_frame(gpu, dims)
+    else:
+        dims, w = pins.shading_world()
+        f = {"range": lambda: pins.light_range_frame(gpu, dims),
+             "overbright": lambda: pins.overbright_frame(gpu, dims),
+             "falloff27": lambda: pins.falloff_frame(gpu, dims, float(2 ** 27)),
+             "falloff28": lambda: pins.falloff_frame(gpu, dims, float(2 ** 28))}[case]()
+    ctx = gpu.Context(dims)
+    ctx.upload(w)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    r = orc.render(wdf, dims, f, 8, 8)
+    for cutoff in (False, True):
+        ctx.set_shadow_cutoff(cutoff)
+        t = gpu.Target(8, 8, depth=True, aux=True)
+        ctx.stats(reset=True)
+        ctx.draw(t, frame=f)
+        g = t.download()
+        g.update(ctx.stats(reset=True))
+        g["shadow_cutoff"] = cutoff
+        assert_frames_match(g, r, "%s cutoff=%d" % (case, cutoff))
+        assert np.array_equal(g["rgb"], r["rgb"]), case  # exact lighting terms: no rounding slack needed
+    ctx.close()

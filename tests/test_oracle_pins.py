@@ -261,6 +261,91 @@ def test_depth_field_jump_skips_empty_space(orc):
     assert r["hit"][0] == -1 and 9 <= r["steps"][0] <= 10
 
 
+def truncation_rays():
+    """Rays that start outside the grid, in (-1, 0) and (-2, -1) on x.  GLSL's
+    int(float) drops the fraction (GLSL 4.60 5.4.1), so x = -0.5 lies in cell
+    0 and x = -1.5 in cell -1; the start cell is never tested."""
+    dims = (8, 4, 4)
+    w = empty_grid(dims)
+    put(w, dims, 0, 1, 1, 0x111111)
+    put(w, dims, 1, 1, 1, 0x222222)
+    d = np.array([1.0, 1e-4, 2e-4], np.float32)
+    return dims, w, [[-0.5, 1.5, 1.5], [-1.5, 1.5, 1.5]], [d, d]
+
+
+def test_start_cell_truncates_toward_zero(orc):
+    dims, w, start, d = truncation_rays()
+    r = orc.cast_rays(w, dims, start, d, [384, 384], 384)
+    assert r["hit"][0] == idx(dims, 1, 1, 1) and r["steps"][0] == 1  # cell 0 is the (untested) start
+    assert r["hit"][1] == idx(dims, 0, 1, 1) and r["steps"][1] == 1  # cell -1 -> 0
+
+
+def zero_component_rays():
+    """Rays whose direction has an exactly zero component.  sign(0) = 0
+    (GLSL 4.60 8.3), so that axis never moves and its first boundary
+    distance is (cell - start) / dir with no forward step:
+      * dir.y = +0 and a fractional start: -inf, the smallest, so y is the
+        axis taken on every iteration and the ray never leaves its cell;
+      * dir.y = -0: +inf, so y is never taken and the ray walks in x and z;
+      * dir.y = +0 and start.y integral: 0/0 = NaN, every comparison with it
+        is false, so z (the last branch) is taken on every iteration."""
+    dims = (8, 4, 8)
+    w = empty_grid(dims)
+    for x in range(dims[0]):
+        for y in range(dims[1]):
+            put(w, dims, x, y, 6, 0x336699)  # a wall across z = 6
+    d_pos = np.array([0.6, 0.0, 0.8], np.float32)
+    d_neg = np.array([0.6, -0.0, 0.8], np.float32)
+    start = [[0.5, 1.5, 0.5], [0.5, 1.5, 0.5], [0.5, 1.0, 0.5]]
+    return dims, w, start, [d_pos, d_neg, d_pos]
+
+
+def test_zero_direction_component(orc):
+    dims, w, start, d = zero_component_rays()
+    assert math.copysign(1.0, d[1][1]) < 0
+    r = orc.cast_rays(w, dims, start, d, [384, 384, 384], 384)
+    assert r["hit"][0] == -1 and r["steps"][0] == 384  # stuck in its cell until the budget ends
+    # -0: an ordinary walk in the xz plane from (0.5, 0.5) along (0.6, 0.8)
+    e = exact_first_solid(w, dims, [0.5, 1.5, 0.5], [0.6, 1e-9, 0.8], 384)
+    assert (r["hit"][1], r["steps"][1]) == e and r["hit"][1] >= 0
+    assert r["hit"][2] == idx(dims, 0, 1, 6) and r["steps"][2] == 6  # straight along z
+
+
+def step_bias_rays():
+    """fshader.glsl:79-81 adds 1e-6 to each direction component before |.|
+    in the step sizes, so a step along +0.5 is 1/0.500001 = 1.999996 and one
+    along -0.5 is 1/0.499999 = 2.000004 (2 without the bias).  Each ray is
+    built so that its first y boundary lies between the biased and the
+    unbiased second x boundary, 1 + step:
+      A: dir (+0.5, 0.25), first y boundary at 4 * (1 - 0.2500005) = 2.999998:
+         1 + 1.999996 is smaller, so x is taken twice and the ray enters
+         (2, 0, 0); unbiased it would take y and enter (1, 1, 0);
+      B: dir (-0.5, 0.25), first y boundary at 4 * (1 - 0.2499995) = 3.000002:
+         1 + 2.000004 is larger, so y comes second and the ray enters
+         (6, 1, 0); unbiased it would take x again and enter (5, 0, 0)."""
+    dims = (8, 4, 2)
+    w = empty_grid(dims)
+    for c in [(2, 0, 0), (1, 1, 0), (6, 1, 0), (5, 0, 0)]:
+        put(w, dims, *c, 0x445566)
+    start = [[0.5, 0.2500005, 0.5], [7.5, 0.2499995, 0.5]]
+    d = [np.array([0.5, 0.25, 1e-4], np.float32), np.array([-0.5, 0.25, 1e-4], np.float32)]
+    return dims, w, start, d
+
+
+def test_step_bias_shortens_positive_steps(orc):
+    dims, w, start, d = step_bias_rays()
+    f = np.float32
+    one = f(1)
+    step_pos = one / abs(f(0.5) + f(1e-6))
+    step_neg = one / abs(f(-0.5) + f(1e-6))
+    y_a = (one - f(start[0][1])) / f(0.25)
+    y_b = (one - f(start[1][1])) / f(0.25)
+    assert one + step_pos < y_a < one + one + one < y_b < one + step_neg  # the premise above
+    r = orc.cast_rays(w, dims, start, d, [384, 384], 384)
+    assert r["hit"][0] == idx(dims, 2, 0, 0) and r["steps"][0] == 2
+    assert r["hit"][1] == idx(dims, 6, 1, 0) and r["steps"][1] == 2
+
+
 # ----------------------------------------------------------- shading closed forms
 def shading_world():
     dims = (32, 32, 32)
@@ -325,6 +410,118 @@ def test_overbright_caps_at_1_25(orc, vrt):
     assert np.all(r["rgb"].reshape(-1, 3) == expect)
 
 
+# Lights 2^26 cells straight above the floor's top face (y = 5) make the
+# lighting terms exact: from a floor point the offset to the light is
+# (lx, 2^26, lz) with |lx|, |lz| < 16 once 2^26 + 8 - 5 is rounded to float
+# (spacing 8 there), lx^2 + 2^52 + lz^2 rounds to 2^52, so the distance is
+# 2^26, the direction is (lx / 2^26, 1, lz / 2^26) and N.L = 1 exactly.
+HIGH = float(2 ** 26 + 8)
+
+
+def high_light(L, x, radius, rgb, z=16.0):
+    L.pos[:] = [x, HIGH, z]
+    L.radius = radius
+    L.rgb[:] = [rgb, rgb, rgb]
+
+
+def lit(c, m):
+    return unorm8(np.float32(np.float32(c) / np.float32(255.0)) * np.float32(m))
+
+
+def light_range_frame(vrt, dims):
+    """Sun overhead (multiplier 0.4 + 0.8 = 1.2); lights whose distance 2^26
+    equals, exceeds and falls short of their radius."""
+    f = down_frame(vrt, dims, [16.0, HIGH, 16.0])
+    high_light(f.lights[0], 16.0, float(2 ** 26), 0.0)      # d == r: in range, falloff 0
+    high_light(f.lights[1], 16.0, float(2 ** 26 - 4), 0.0)  # d > r: skipped
+    high_light(f.lights[2], 0.0, float(2 ** 60), 0.0)       # x == 0 is a placed light (unused slots hold -1)
+    return f
+
+
+def test_light_at_exactly_its_radius_is_evaluated(orc, vrt):
+    """fshader.glsl:167 tests localLightDist <= radius and :162 pos >= 0."""
+    dims, w = shading_world()
+    wdf = orc.depth_field(w, dims)
+    r = orc.render(wdf, dims, light_range_frame(vrt, dims), 8, 8)
+    assert np.all(r["hit"] >= 0)
+    assert np.all(r["shadow_eval"] == 0b1011) and np.all(r["shadow_occ"] == 0)
+    m = np.float32(0.4) + np.float32(0.8)
+    assert np.all(r["rgb"].reshape(-1, 3) == [lit(c, m) for c in (200, 100, 50)])
+
+
+def overbright_frame(vrt, dims):
+    """Sun overhead gives 0.4 + 0.8 = 1.2 and light 0 (falloff 1, since
+    2^60 - 2^26 rounds to 2^60) adds 0.05: float32 rounds 1.2 + 0.05 to
+    exactly 1.25, so the >= 1.25 test at the top of the next iteration
+    (fshader.glsl:158) stops the loop before light 1."""
+    f = down_frame(vrt, dims, [16.0, HIGH, 16.0])
+    for i in range(3):
+        high_light(f.lights[i], 16.0, float(2 ** 60), 0.05)
+    return f
+
+
+def test_overbright_stops_at_exactly_1_25(orc, vrt):
+    f32 = np.float32
+    assert (f32(0.4) + f32(0.8)) + f32(0.05) == f32(1.25)  # the premise above
+    dims, w = shading_world()
+    wdf = orc.depth_field(w, dims)
+    r = orc.render(wdf, dims, overbright_frame(vrt, dims), 8, 8)
+    assert np.all(r["shadow_eval"] == 0b11) and np.all(r["shadow_occ"] == 0)
+    assert np.all(r["rgb"].reshape(-1, 3) == [lit(c, 1.25) for c in (200, 100, 50)])
+
+
+def falloff_frame(vrt, dims, radius):
+    """Sun under the floor (lambert 0, multiplier stays 0.4) and one light of
+    colour 0.5 at distance 2^26 with radius `radius`."""
+    f = down_frame(vrt, dims, [16.0, -500.0, 16.0])
+    high_light(f.lights[0], 16.0, radius, 0.5)
+    return f
+
+
+@pytest.mark.parametrize("log_r,fall", [(27, 0.5), (28, 0.75), (30, 0.9375)])
+def test_light_falloff_is_linear_in_distance(orc, vrt, log_r, fall):
+    """Colour * N.L * (radius - d) / radius (fshader.glsl:170-177) with
+    N.L = 1 and d = 2^26: exactly 1 - 2^(26 - log_r)."""
+    dims, w = shading_world()
+    wdf = orc.depth_field(w, dims)
+    r = orc.render(wdf, dims, falloff_frame(vrt, dims, float(2 ** log_r)), 8, 8)
+    assert np.all(r["shadow_eval"] == 0b11) and np.all(r["shadow_occ"] == 0b00)
+    m = np.float32(0.4) + np.float32(0.5 * fall)
+    assert np.all(r["rgb"].reshape(-1, 3) == [lit(c, m) for c in (200, 100, 50)])
+
+
+def roof_world():
+    dims, w = shading_world()
+    for x in range(32):
+        for z in range(32):
+            put(w, dims, x, 28, z, 0x808080)  # a roof at y = 28 over the floor at y = 4
+    return dims, w
+
+
+def roof_frame(vrt, dims):
+    """Camera (y = 20.7) between floor and roof.  The sun overhead is behind
+    the roof.  Light 0 (y = 12) is under the roof: its shadow ray gets
+    int(d + 1) iterations (fshader.glsl:175), d < 14 here, and reaching the
+    roof takes at least 23 (one per y boundary from 5 to 28), so the roof
+    cannot shadow it.  Light 1, 2^26 up, is behind the roof."""
+    f = down_frame(vrt, dims, [16.0, 1e6, 16.0])
+    L = f.lights[0]
+    L.pos[:] = [16.0, 12.0, 16.0]
+    L.radius = 64.0
+    L.rgb[:] = [0.1, 0.1, 0.1]
+    high_light(f.lights[1], 16.0, float(2 ** 60), 0.1)
+    return f
+
+
+def test_local_shadow_rays_stop_at_the_light(orc, vrt):
+    dims, w = roof_world()
+    wdf = orc.depth_field(w, dims)
+    r = orc.render(wdf, dims, roof_frame(vrt, dims), 8, 8)
+    assert np.all(r["hit"] >= 0) and np.all(r["face"] == 1 * 3 + 0)  # all on the floor
+    assert np.all(r["shadow_eval"] == 0b111)
+    assert np.all(r["shadow_occ"] == 0b101)  # sun and light 1 behind the roof, light 0 not
+
+
 def test_depth_view_is_step_count(orc, vrt):
     dims, w = shading_world()
     wdf = orc.depth_field(w, dims)
@@ -386,9 +583,13 @@ def test_depth_field_values_are_cube_distances(orc):
                     assert bits_f32(int(got)) == np.float32(-best)
 
 
-def test_remove_sphere_equals_rebuild_inside_repair_radius(orc):
+@pytest.mark.parametrize("p", [0.25, 0.02])
+def test_remove_sphere_equals_rebuild_inside_repair_radius(orc, p):
+    """removeSphere (level.cpp:30-56) carves radius r and repairs the depth
+    field within r + 3; there it must equal a full rebuild.  The sparse world
+    has cells beyond r whose nearest solid was carved."""
     dims = (40, 24, 36)
-    w = small_world(4, dims, p=0.25)
+    w = small_world(4, dims, p=p)
     df = orc.depth_field(w, dims)
     before = df.copy()
     c, r = (18, 10, 17), 6
@@ -415,6 +616,10 @@ def test_remove_sphere_equals_rebuild_inside_repair_radius(orc):
     for k in np.nonzero(before < 0)[0]:
         assert jump(df[k]) >= jump(before[k])
     assert np.all((df >= 0) <= (before >= 0))  # solids only disappear
+    if p < 0.1:
+        shell = [idx(dims, c[0] + x, c[1] + y, c[2] + z) for z in range(-R, R) for y in range(-R, R)
+                 for x in range(-R, R) if r * r <= x * x + y * y + z * z < R * R]
+        assert np.count_nonzero(df[shell] != before[shell]) > 50  # the repair reached past the carve
 
 
 # ------------------------------------------------------------ reference world
