@@ -307,6 +307,8 @@ void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut
                 }
             }
 
+            if (MUT(24) && multiplier[0] >= 1.25f && multiplier[1] >= 1.25f && multiplier[2] >= 1.25f)
+                multiplier[0] = multiplier[1] = multiplier[2] = 1.25f;  /* MUT(24): clamp after the last light too */
             int32_t v = g.vox[fColorIndex];
             if (MUT(19)) v = ((v & 0xFF) << 16) | (v & 0xFF00) | ((v >> 16) & 0xFF);  /* red and blue swapped */
             fColor[0] = (float)((v & 0x00FF0000) >> 16) / 255.0f * multiplier[0];

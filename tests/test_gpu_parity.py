@@ -395,11 +395,13 @@ def test_pin_rays_match_oracle(gpu, orc):
         ctx.close()
 
 
-@pytest.mark.parametrize("case", ["range", "overbright", "falloff27", "falloff28", "roof"])
+@pytest.mark.parametrize("case", ["range", "overbright", "falloff27", "falloff28", "roof", "clamp0", "clamp15",
+                                  "clamp14_15"])
 def test_pin_frames_match_oracle(gpu, orc, case):
     """The exact local-light scenes that pin the oracle (light at exactly its
     radius, overbright reached at exactly 1.25, linear falloff, shadow rays
-    that stop at the light) through the kernels, cutoff off and on."""
+    that stop at the light, the clamp after the first, the last and the
+    next-to-last light slot) through the kernels, cutoff off and on."""
     import test_oracle_pins as pins
     if case == "roof":
         dims, w = pins.roof_world()
@@ -409,7 +411,10 @@ def test_pin_frames_match_oracle(gpu, orc, case):
         f = {"range": lambda: pins.light_range_frame(gpu, dims),
              "overbright": lambda: pins.overbright_frame(gpu, dims),
              "falloff27": lambda: pins.falloff_frame(gpu, dims, float(2 ** 27)),
-             "falloff28": lambda: pins.falloff_frame(gpu, dims, float(2 ** 28))}[case]()
+             "falloff28": lambda: pins.falloff_frame(gpu, dims, float(2 ** 28)),
+             "clamp0": lambda: pins.clamp_frame(gpu, dims, (0,)),
+             "clamp15": lambda: pins.clamp_frame(gpu, dims, (15,)),
+             "clamp14_15": lambda: pins.clamp_frame(gpu, dims, (14, 15))}[case]()
     ctx = gpu.Context(dims)
     ctx.upload(w)
     ctx.build_depth_field()

@@ -43,6 +43,7 @@ MUTANTS = {
     21: "removeSphere repairs only the carved ball",
     22: "depth view scaled by 1/255 instead of 1/100",
     23: "aspect ratio applied to y",
+    24: "overbright clamp applied after the last light too",
 }
 
 

@@ -470,6 +470,36 @@ def test_overbright_stops_at_exactly_1_25(orc, vrt):
     assert np.all(r["rgb"].reshape(-1, 3) == [lit(c, 1.25) for c in (200, 100, 50)])
 
 
+def clamp_frame(vrt, dims, slots):
+    """Sun overhead (1.2) and lights of colour 0.5 with falloff 1 and N.L = 1
+    in the given slots only (the others unused, -1): the first one lifts the
+    multiplier to 1.7."""
+    f = down_frame(vrt, dims, [16.0, HIGH, 16.0])
+    for i in slots:
+        high_light(f.lights[i], 16.0, float(2 ** 60), 0.5)
+    return f
+
+
+@pytest.mark.parametrize("slots,m,evaluated", [((0,), 1.25, 0b11), ((15,), 1.7, 0b1 | 2 << 15),
+                                               ((14, 15), 1.25, 0b1 | 2 << 14)])
+def test_overbright_clamp_needs_a_following_iteration(orc, vrt, slots, m, evaluated):
+    """The clamp to 1.25 is the first statement of the light loop
+    (fshader.glsl:158-164), so it happens only if an iteration follows the
+    light that crossed 1.25, placed or not: after slot 0 it does (slot 1's
+    iteration clamps), after slot 15 it does not (no clamp after the last
+    light); with slots 14 and 15 placed, slot 15's iteration clamps and
+    stops before evaluating it."""
+    f32 = np.float32
+    m_after = (f32(0.4) + f32(0.8)) + f32(0.5)
+    assert m_after > f32(1.25)
+    dims, w = shading_world()
+    wdf = orc.depth_field(w, dims)
+    r = orc.render(wdf, dims, clamp_frame(vrt, dims, slots), 8, 8)
+    assert np.all(r["shadow_eval"] == evaluated) and np.all(r["shadow_occ"] == 0)
+    mm = f32(1.25) if m == 1.25 else m_after
+    assert np.all(r["rgb"].reshape(-1, 3) == [lit(c, mm) for c in (200, 100, 50)])
+
+
 def falloff_frame(vrt, dims, radius):
     """Sun under the floor (lambert 0, multiplier stays 0.4) and one light of
     colour 0.5 at distance 2^26 with radius `radius`."""

@@ -2,7 +2,7 @@
 // without a window: draw, lightUpdate, scripted mouse look and movement,
 // destroy edits, placeLocalLight, the depth-view toggle, updateUniforms.
 //
-//   vrt_headless [--frames N] [--width W] [--height H] [--out DIR] [--record]
+//   vrt_headless [--frames N] [--width W] [--height H] [--out DIR] [--record] [--entity]
 //
 // World: the reference's initVoxels (512 x 96 x 512) with its depth field,
 // built on the device.  Input is scripted (fps fixed at 60): the mouse swings
@@ -12,6 +12,14 @@
 // (SHIFT).  With --record every frame's constants, RGB8 and hit map go to
 // DIR/frame_NNNN.bin and the edits to DIR/edits.txt; the last frame is also
 // written as DIR/last.ppm.  Prints frame timings and Mrays/s.
+//
+// --entity turns on the reference's test entity (commented out in
+// src/main.cpp:48-50,67-70; src/Entity.cpp): a 10^3 box at (195, 50, 165)
+// whose centre voxel moves +z by 2/fps a frame (Entity::move: destroyVoxel,
+// then placeVoxel 0xAA0000 on the host array) and whose box is re-uploaded
+// every frame (Entity::update: updatePartialGeometry).  The host array is
+// kept equal to the device grid as the reference's is: it is refreshed from
+// the device after the depth-field build and after each sphere carve.
 #include <hip/hip_runtime_api.h>
 
 #include <cmath>
@@ -77,6 +85,31 @@ void move_forward(Camera& c, float fps) {
     c.pos[2] = std::fmin(std::fmax(c.pos[2], 1.0f), 512.0f - 1.0f - 1.0f);
 }
 
+// The reference's test entity (src/Entity.cpp:3-39).
+struct Entity {
+    float x = 195.0f, y = 50.0f, z = 165.0f;  // setPos(195, 50, 165) (src/main.cpp:49)
+    int xdim = 10, ydim = 10, zdim = 10;      // new Entity(10, 10, 10) (src/main.cpp:48)
+
+    // move(): destroyVoxel at the centre, z += 2/fps, placeVoxel 0x00AA0000
+    // at the new centre, on the host array (src/render.cpp:256-271 bounds).
+    void move(int32_t* vox, int64_t X, int64_t Y, int64_t Z, float fps, int from[3], int to[3]) {
+        from[0] = (int)x + xdim / 2; from[1] = (int)y + ydim / 2; from[2] = (int)z + zdim / 2;
+        set(vox, X, Y, Z, from, -1);
+        z += 2.0f / fps;
+        to[0] = (int)x + xdim / 2; to[1] = (int)y + ydim / 2; to[2] = (int)z + zdim / 2;
+        set(vox, X, Y, Z, to, 0x00AA0000);
+    }
+    // update(): updatePartialGeometry(pos, pos + dims), corners truncated.
+    void box(int lo[3], int hi[3]) const {
+        lo[0] = (int)x; lo[1] = (int)y; lo[2] = (int)z;
+        hi[0] = (int)(x + (float)xdim); hi[1] = (int)(y + (float)ydim); hi[2] = (int)(z + (float)zdim);
+    }
+    static void set(int32_t* vox, int64_t X, int64_t Y, int64_t Z, const int c[3], int32_t v) {
+        if (c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < X && c[1] < Y && c[2] < Z)
+            vox[c[0] + X * c[1] + X * Y * c[2]] = v;
+    }
+};
+
 void write_ppm(const std::string& path, const std::vector<uint8_t>& rgb, int w, int h) {
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) return;
@@ -90,6 +123,7 @@ void write_ppm(const std::string& path, const std::vector<uint8_t>& rgb, int w, 
 int main(int argc, char** argv) {
     int frames = 32, width = 800, height = 600;
     bool record = false;
+    bool entity = false;
     std::string out = ".";
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
@@ -98,8 +132,10 @@ int main(int argc, char** argv) {
         else if (a == "--height" && i + 1 < argc) height = std::atoi(argv[++i]);
         else if (a == "--out" && i + 1 < argc) out = argv[++i];
         else if (a == "--record") record = true;
+        else if (a == "--entity") entity = true;
         else {
-            std::fprintf(stderr, "usage: %s [--frames N] [--width W] [--height H] [--out DIR] [--record]\n", argv[0]);
+            std::fprintf(stderr, "usage: %s [--frames N] [--width W] [--height H] [--out DIR] [--record] [--entity]\n",
+                         argv[0]);
             return 2;
         }
     }
@@ -110,6 +146,8 @@ int main(int argc, char** argv) {
     check(vrt_world_reference(X, Y, Z, voxels.data()), "vrt_world_reference");  // initVoxels
     check(vrt_upload(ctx, voxels.data()), "vrt_upload");
     check(vrt_build_depth_field(ctx), "vrt_build_depth_field");
+    if (entity) check(vrt_download(ctx, voxels.data()), "vrt_download");
+    Entity ent;
 
     vrt_frame frame;
     check(vrt_frame_defaults(&frame, X, width, height), "vrt_frame_defaults");
@@ -178,6 +216,16 @@ int main(int argc, char** argv) {
             const int r = (int)(range / 2);
             check(vrt_remove_sphere(ctx, c[0], c[1], c[2], r), "vrt_remove_sphere");
             if (edits) std::fprintf(edits, "%d %d %d %d %d\n", k, c[0], c[1], c[2], r);
+            if (entity) check(vrt_download(ctx, voxels.data()), "vrt_download");
+        }
+        if (entity) {  // testE->move(); testE->update();
+            int from[3], to[3];
+            ent.move(voxels.data(), X, Y, Z, fps, from, to);
+            int lo[3], hi[3];
+            ent.box(lo, hi);
+            check(vrt_upload_region(ctx, voxels.data(), lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]), "vrt_upload_region");
+            if (edits)
+                std::fprintf(edits, "e %d %d %d %d %d %d %d\n", k, from[0], from[1], from[2], to[0], to[1], to[2]);
         }
         // updateUniforms (src/render.cpp:289-302)
         std::memcpy(frame.cam_pos, cam.pos, sizeof(cam.pos));

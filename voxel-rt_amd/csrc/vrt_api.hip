@@ -90,7 +90,12 @@ FrameDesc make_frame_desc(const vrt_frame& f) {
     FrameDesc d;
     d.f = f;
     d.render_lim = (float)f.render_dist;
-    for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) d.light_dd_max[i] = sqrt_le_limit(f.lights[i].radius);
+    d.light_valid = 0;
+    for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
+        const vrt_local_light& L = f.lights[i];
+        d.light_dd_max[i] = sqrt_le_limit(L.radius);
+        if (L.pos[0] >= 0.0f && L.pos[1] >= 0.0f && L.pos[2] >= 0.0f) d.light_valid |= 1u << i;
+    }
     return d;
 }
 

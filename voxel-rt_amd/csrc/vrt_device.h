@@ -323,16 +323,24 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             }
         }
 
-        // local lights (:158-181)
-        for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
+        // local lights (:158-181).  The loop's first statement clamps to
+        // MAX_OVERBRIGHT and leaves once every channel is >= 1.25; the
+        // channels change only when a light adds to them, so the test is made
+        // after each addition (and once for the sun) instead of on every
+        // iteration, and the clamp is applied where the next iteration's test
+        // would apply it: always after the sun, after light i only if i < 15
+        // (no iteration follows the last).  Unplaced lights do nothing but
+        // that test, so the loop visits the placed ones only.
+        bool over = (m0 >= 1.25f) & (m1 >= 1.25f) & (m2 >= 1.25f);
+        if (over) {
+            m0 = 1.25f;
+            m1 = 1.25f;
+            m2 = 1.25f;
+        }
+        for (uint32_t rem = F.light_valid; rem != 0u; rem &= rem - 1u) {
+            if (over) break;
+            const int i = __builtin_ctz(rem);
             const vrt_local_light& L = f.lights[i];
-            if ((m0 >= 1.25f) & (m1 >= 1.25f) & (m2 >= 1.25f)) {  // MAX_OVERBRIGHT
-                m0 = 1.25f;
-                m1 = 1.25f;
-                m2 = 1.25f;
-                break;
-            }
-            if (!((L.pos[0] >= 0.0f) & (L.pos[1] >= 0.0f) & (L.pos[2] >= 0.0f))) continue;
             const float vx0 = L.pos[0] - hx, vy0 = L.pos[1] - hy, vz0 = L.pos[2] - hz;
             const float dd = dot3(vx0, vy0, vz0, vx0, vy0, vz0);
             // length <= radius (:171), decided without the square root
@@ -354,6 +362,12 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
                 m0 += L.rgb[0] * lam * fall;
                 m1 += L.rgb[1] * lam * fall;
                 m2 += L.rgb[2] * lam * fall;
+                over = (m0 >= 1.25f) & (m1 >= 1.25f) & (m2 >= 1.25f);
+                if (over & (i < VRT_MAX_LOCAL_LIGHTS - 1)) {
+                    m0 = 1.25f;
+                    m1 = 1.25f;
+                    m2 = 1.25f;
+                }
             } else {
                 occ |= 2u << i;
             }

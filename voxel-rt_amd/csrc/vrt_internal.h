@@ -44,6 +44,9 @@ struct FrameDesc {
     // Per light: the largest float t with sqrt_rn(t) <= radius, so that
     // length(v) <= radius  <=>  dot(v, v) <= t  (sqrt_rn is monotone).
     float light_dd_max[VRT_MAX_LOCAL_LIGHTS];
+    // Bit i set when light i is placed: every coordinate >= 0 (unused slots
+    // hold -1, src/render.cpp:304-311; NaN compares false).
+    uint32_t light_valid;
 };
 
 // Ray/step counters: kCounterSlots slots of one 128-byte line each
