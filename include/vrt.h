@@ -310,6 +310,12 @@ double vrt_server_clock(vrt_server* s);
  * and d = 1 + j/2^23, j in [d_begin, d_begin + d_count): all significand
  * pairs when run over j in [0, 2^23).  Returns the mismatch count. */
 int vrt_selftest_division(vrt_ctx* ctx, uint32_t d_begin, uint32_t d_count, uint64_t* mismatches);
+/* Reciprocals (step sizes, normalize, 1/distance) use a hardware reciprocal
+ * refined by one FMA Newton step.  This compares that path with IEEE
+ * division 1/d on this device for the float bit patterns
+ * [first, first + count) (clipped to 2^32): every float when run over
+ * [0, 2^32).  Returns the mismatch count (any NaN equals any NaN). */
+int vrt_selftest_reciprocal(vrt_ctx* ctx, uint64_t first, uint64_t count, uint64_t* mismatches);
 
 /* ---- synthetic worlds (benchmark inputs; no reference counterpart) ------ */
 

@@ -1,6 +1,7 @@
 """The DDA's division by a ray's direction component uses a reciprocal plus
-one FMA correction (Markstein) instead of an IEEE division; these check it
-equals IEEE division bit for bit."""
+one FMA correction (Markstein) instead of an IEEE division, and reciprocals
+use the hardware reciprocal plus one FMA Newton step; these check both equal
+IEEE division bit for bit."""
 import ctypes as C
 import os
 
@@ -40,4 +41,12 @@ def test_gpu_all_numerators_for_spread_divisors(gpu):
         bad += ctx.selftest_division(b * 8192, 64)
     bad += ctx.selftest_division((1 << 23) - 64, 64)
     assert bad == 0
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_reciprocal_every_float(gpu):
+    """The reciprocal path against IEEE 1/d for all 2^32 float bit patterns."""
+    ctx = gpu.Context((8, 8, 8))
+    assert ctx.selftest_reciprocal(0, 1 << 32) == 0
     ctx.close()

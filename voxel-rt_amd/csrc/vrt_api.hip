@@ -86,14 +86,17 @@ float sqrt_le_limit(float r) {
     return t;
 }
 
-FrameDesc make_frame_desc(const vrt_frame& f) {
+FrameDesc make_frame_desc(const vrt_frame& f, int32_t width, int32_t height) {
     FrameDesc d;
     d.f = f;
     d.render_lim = (float)f.render_dist;
     d.light_valid = 0;
+    d.inv_w = width > 0 ? 1.0f / (float)width : 0.0f;
+    d.inv_h = height > 0 ? 1.0f / (float)height : 0.0f;
     for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
         const vrt_local_light& L = f.lights[i];
         d.light_dd_max[i] = sqrt_le_limit(L.radius);
+        d.light_rinv[i] = 1.0f / L.radius;
         if (L.pos[0] >= 0.0f && L.pos[1] >= 0.0f && L.pos[2] >= 0.0f) d.light_valid |= 1u << i;
     }
     return d;
@@ -543,7 +546,7 @@ int vrt_draw_frame(vrt_ctx* ctx, const vrt_frame* f, const vrt_target* t) {
     if (int rc = check_frame(*f)) return rc;
     if (int rc = check_target(*t)) return rc;
     ctx->draws++;
-    return launch_render_plain(render_grid(ctx), make_frame_desc(*f), *t, ctx->counters, ctx->stream);
+    return launch_render_plain(render_grid(ctx), make_frame_desc(*f, t->width, t->height), *t, ctx->counters, ctx->stream);
 }
 
 // glDrawArrays over the fullscreen quad, src/main.cpp:59.
@@ -582,6 +585,25 @@ int vrt_cast_rays(vrt_ctx* ctx, int64_t n, const float* start, const float* dir,
     if (n < 0 || (n > 0 && (!start || !dir || !dist))) return fail(VRT_ERR_INVALID, "invalid ray arrays");
     if (render_dist < 0 || render_dist > (1 << 24)) return fail(VRT_ERR_INVALID, "render_dist must be in [0, 2^24]");
     return launch_cast_rays(ctx->grid, n, start, dir, dist, render_dist, hit, hit_pos, face, steps, ctx->stream);
+}
+
+int vrt_selftest_reciprocal(vrt_ctx* ctx, uint64_t first, uint64_t count, uint64_t* mismatches) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!mismatches || first >= (1ull << 32)) return fail(VRT_ERR_INVALID, "invalid selftest range");
+    unsigned long long* bad = nullptr;
+    VRT_HIP(hipMalloc(&bad, sizeof(unsigned long long)));
+    hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned long long), ctx->stream);
+    int rc = e == hipSuccess ? selftest_reciprocal(first, count, bad, ctx->stream) : hip_fail(e, "hipMemsetAsync");
+    unsigned long long h = 0;
+    if (!rc) {
+        e = hipMemcpyAsync(&h, bad, sizeof(h), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "selftest copy");
+    }
+    hipFree(bad);
+    if (rc) return rc;
+    *mismatches = h;
+    return VRT_OK;
 }
 
 int vrt_selftest_division(vrt_ctx* ctx, uint32_t d_begin, uint32_t d_count, uint64_t* mismatches) {

@@ -77,6 +77,26 @@ __device__ __forceinline__ bool div_range_ok(float d) {
     return (ud - 0x21800000u) <= (0x5d800000u - 0x21800000u);
 }
 
+// a / d for a = +0 or an integer in [1, 2^25] and d in [1, 2^24] with
+// y = RN(1/d): the correction of div_by with every quantity normal (or +0),
+// so no range test is needed.
+__device__ __forceinline__ float div_small_int(float a, float d, float y) {
+    const float q0 = a * y;
+    return __builtin_fmaf(__builtin_fmaf(-q0, d, a), y, q0);
+}
+
+// RN(1/d): v_rcp_f32 (within one ulp) refined by one Newton step with FMAs,
+// e = RN(1 - d*y0), y = RN(y0 + y0*e).  For |d| in [2^-60, 2^60] this equals
+// IEEE division 1/d for every input: the library's reciprocal selftest
+// compares the two for all 2^32 bit patterns.  Other inputs take the
+// division itself.
+__device__ __forceinline__ float rcp_rn(float d) {
+    const float y0 = __builtin_amdgcn_rcpf(d);
+    float y = __builtin_fmaf(__builtin_fmaf(-d, y0, 1.0f), y0, y0);
+    if (__builtin_expect(!div_range_ok(d), 0)) y = 1.0f / d;
+    return y;
+}
+
 // dot, length, normalize in GLM 1.0.0's order (func_geometric.inl:48-54,88;
 // func_exponential.inl:136-139).
 __device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
@@ -155,10 +175,10 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
     const int32_t stx = isign(dx), sty = isign(dy), stz = isign(dz);
     const int32_t fx = stx > 0, fy = sty > 0, fz = stz > 0;
-    const float ddx = 1.0f / fabsf(dx + 0.000001f);
-    const float ddy = 1.0f / fabsf(dy + 0.000001f);
-    const float ddz = 1.0f / fabsf(dz + 0.000001f);
-    const float rx = 1.0f / dx, ry = 1.0f / dy, rz = 1.0f / dz;
+    const float ddx = rcp_rn(fabsf(dx + 0.000001f));
+    const float ddy = rcp_rn(fabsf(dy + 0.000001f));
+    const float ddz = rcp_rn(fabsf(dz + 0.000001f));
+    const float rx = rcp_rn(dx), ry = rcp_rn(dy), rz = rcp_rn(dz);
     const bool dok = (int(div_range_ok(dx)) & int(div_range_ok(dy)) & int(div_range_ok(dz))) != 0;
     float ix, iy, iz;
     div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok, ix, iy,
@@ -262,7 +282,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
 
     // normalize(vec3(vPos.x*aspectRatio, vPos.y, focal)) (:136)
     const float ux = vx * f.aspect_ratio, uy = vy, uz = f.focal;
-    const float un = 1.0f / sqrtf(dot3(ux, uy, uz, ux, uy, uz));
+    const float un = rcp_rn(sqrtf(dot3(ux, uy, uz, ux, uy, uz)));
     const float rx = ux * un, ry = uy * un, rz = uz * un;
     // vec3(rotateMatrix * vec4(d, 0)) (:137), GLM's pairwise column sum
     const float* M = f.rotate_matrix;
@@ -293,7 +313,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
     }
 
     if (f.view_depth_field == 1) {
-        const float s = (float)h.iters / 100.0f;
+        const float s = div_small_int((float)h.iters, 100.0f, 1.0f / 100.0f);
         col0 = s;
         col1 = s;
         col2 = s;
@@ -301,7 +321,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
         const int32_t vcol = g.vox[h.index];
         // toLight = normalize(lightPos - firstHitPos) (:147)
         const float lx0 = f.light_pos[0] - hx, ly0 = f.light_pos[1] - hy, lz0 = f.light_pos[2] - hz;
-        const float ln = 1.0f / sqrtf(dot3(lx0, ly0, lz0, lx0, ly0, lz0));
+        const float ln = rcp_rn(sqrtf(dot3(lx0, ly0, lz0, lx0, ly0, lz0)));
         const float tlx = lx0 * ln, tly = ly0 * ln, tlz = lz0 * ln;
         float m0 = 0.4f, m1 = 0.4f, m2 = 0.4f;  // AMBIENT
 
@@ -346,7 +366,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             // length <= radius (:171), decided without the square root
             if (!(dd <= F.light_dd_max[i])) continue;
             const float dist = sqrtf(dd);
-            const float inv = 1.0f / dist;
+            const float inv = rcp_rn(dist);
             const float ux1 = vx0 * inv, uy1 = vy0 * inv, uz1 = vz0 * inv;
             float lim = (float)f2i(dist + 1.0f);
             lim = lim < F.render_lim ? lim : F.render_lim;
@@ -358,7 +378,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             eval |= 2u << i;
             if (s.index == -1) {
                 const float lam = max0(dot3(nx, ny, nz, ux1, uy1, uz1));
-                const float fall = (L.radius - dist) / L.radius;
+                const float fall = div_by(L.radius - dist, L.radius, F.light_rinv[i], div_range_ok(L.radius));
                 m0 += L.rgb[0] * lam * fall;
                 m1 += L.rgb[1] * lam * fall;
                 m2 += L.rgb[2] * lam * fall;
@@ -373,9 +393,9 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             }
         }
 
-        col0 = (float)((vcol & 0x00FF0000) >> 16) / 255.0f * m0;
-        col1 = (float)((vcol & 0x0000FF00) >> 8) / 255.0f * m1;
-        col2 = (float)(vcol & 0x000000FF) / 255.0f * m2;
+        col0 = div_small_int((float)((vcol & 0x00FF0000) >> 16), 255.0f, 1.0f / 255.0f) * m0;
+        col1 = div_small_int((float)((vcol & 0x0000FF00) >> 8), 255.0f, 1.0f / 255.0f) * m1;
+        col2 = div_small_int((float)(vcol & 0x000000FF), 255.0f, 1.0f / 255.0f) * m2;
     }
     o.rgb[0] = unorm8(col0);
     o.rgb[1] = unorm8(col1);
@@ -399,11 +419,13 @@ __device__ __forceinline__ void store_pixel(const vrt_target& t, int64_t k, cons
     if (t.steps) t.steps[k] = p.steps;
 }
 
-// vPos of pixel (px, row), row 0 at the top (GL rows count from the bottom).
-__device__ __forceinline__ void pixel_vpos(int32_t W, int32_t H, int32_t px, int32_t row, float& vx, float& vy) {
+// vPos of pixel (px, row), row 0 at the top (GL rows count from the bottom):
+// (2px+1)/W - 1 and (2py+1)/H - 1, the divisions through RN(1/W), RN(1/H).
+__device__ __forceinline__ void pixel_vpos(int32_t W, int32_t H, float inv_w, float inv_h, int32_t px, int32_t row,
+                                           float& vx, float& vy) {
     const int32_t py = H - 1 - row;
-    vx = (float)(2 * px + 1) / (float)W - 1.0f;
-    vy = (float)(2 * py + 1) / (float)H - 1.0f;
+    vx = div_small_int((float)(2 * px + 1), (float)W, inv_w) - 1.0f;
+    vy = div_small_int((float)(2 * py + 1), (float)H, inv_h) - 1.0f;
 }
 
 // Sum over the wave, result in every lane.

@@ -47,6 +47,8 @@ struct FrameDesc {
     // Bit i set when light i is placed: every coordinate >= 0 (unused slots
     // hold -1, src/render.cpp:304-311; NaN compares false).
     uint32_t light_valid;
+    float light_rinv[VRT_MAX_LOCAL_LIGHTS];  // RN(1 / radius)
+    float inv_w, inv_h;                      // RN(1 / width), RN(1 / height) of the target
 };
 
 // Ray/step counters: kCounterSlots slots of one 128-byte line each
@@ -88,6 +90,7 @@ int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_
                         unsigned int* queue, unsigned long long* counters, hipStream_t s);
 int launch_queue_reset(unsigned int* queue, hipStream_t s);
 int selftest_division(uint32_t d_begin, uint32_t d_count, unsigned long long* bad_dev, hipStream_t s);
+int selftest_reciprocal(uint64_t first, uint64_t count, unsigned long long* bad_dev, hipStream_t s);
 int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu);
 
 int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, int32_t y0, int32_t z0,
@@ -153,7 +156,7 @@ int solid_top_full(vrt_ctx* c);  // recomputes *grid.top over the whole grid
 
 namespace vrt {
 int world_generate_device(const GridDesc& g, const vrt_world_params& p, hipStream_t s);
-FrameDesc make_frame_desc(const vrt_frame& f);
+FrameDesc make_frame_desc(const vrt_frame& f, int32_t width, int32_t height);
 int check_frame(const vrt_frame& f);
 int check_target(const vrt_target& t);
 }  // namespace vrt

@@ -62,7 +62,8 @@ k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counte
     const uint32_t lb = blockIdx.x;
     const int32_t tiles_x = (t.width + 7) >> 3;
     const int32_t tiles_y = (t.height + 7) >> 3;
-    const int32_t tile = (int32_t)lb * kPlainWaves + (int32_t)(threadIdx.x >> 6);
+    // the wave's tile is uniform: the tile arithmetic runs on the scalar unit
+    const int32_t tile = (int32_t)lb * kPlainWaves + __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
     const int32_t lane = threadIdx.x & 63;
     unsigned long long rays = 0, iters = 0, pixels = 0;
     if (tile < tiles_x * tiles_y) {
@@ -70,7 +71,7 @@ k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counte
         const int32_t row = (tile / tiles_x) * 8 + (lane >> 3);
         if (px < t.width && row < t.height) {
             float vx, vy;
-            dev::pixel_vpos(t.width, t.height, px, row, vx, vy);
+            dev::pixel_vpos(t.width, t.height, F.inv_w, F.inv_h, px, row, vx, vy);
             dev::PixelResult p;
             dev::shade_pixel<IX>(g, F, vx, vy, p, top);
             dev::store_pixel(t, (int64_t)row * t.width + px, p);
@@ -117,7 +118,8 @@ __device__ __forceinline__ void shade_view_tile(const GridDesc& g, const ViewDes
                                                 int32_t n_views, int64_t tile, int32_t tw, int32_t th,
                                                 int32_t lane, int32_t top, uint32_t& rays, uint32_t& iters,
                                                 uint32_t& pixels) {
-    const int32_t rows_per_pass = kWave / tw;
+    const int32_t tw_log2 = __builtin_ctz((uint32_t)tw);  // tw is a power of two
+    const int32_t rows_per_pass = kWave >> tw_log2;
     const int32_t passes = th / rows_per_pass;
     // view owning this tile (uniform across the wave)
     int32_t lo = 0, hi = n_views - 1;
@@ -131,11 +133,11 @@ __device__ __forceinline__ void shade_view_tile(const GridDesc& g, const ViewDes
     const int32_t tx0 = (local % V.tiles_x) * tw;
     const int32_t ty0 = (local / V.tiles_x) * th;
     for (int32_t pass = 0; pass < passes; pass++) {
-        const int32_t px = tx0 + lane % tw;
-        const int32_t row = ty0 + pass * rows_per_pass + lane / tw;
+        const int32_t px = tx0 + (lane & (tw - 1));
+        const int32_t row = ty0 + pass * rows_per_pass + (lane >> tw_log2);
         if (px < V.target.width && row < V.target.height) {
             float vx, vy;
-            dev::pixel_vpos(V.target.width, V.target.height, px, row, vx, vy);
+            dev::pixel_vpos(V.target.width, V.target.height, V.frame.inv_w, V.frame.inv_h, px, row, vx, vy);
             dev::PixelResult p;
             dev::shade_pixel<IX>(g, V.frame, vx, vy, p, top);
             dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
@@ -156,7 +158,7 @@ k_render_views(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, 
                int32_t th, unsigned long long* counters) {
     const int32_t top = g.top ? *g.top : INT32_MAX;
     const int32_t lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t tile = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
     uint32_t rays = 0, iters = 0, pixels = 0;
     if (tile < total_tiles) shade_view_tile<IX>(g, views, n_views, tile, tw, th, lane, top, rays, iters, pixels);
     flush_counters(counters, rays, iters, pixels);
@@ -209,6 +211,22 @@ __global__ void __launch_bounds__(256) k_selftest_div(uint32_t d_begin, uint32_t
         const float d = __uint_as_float(0x3f800000u | j);
         const float y = 1.0f / d;
         n += __float_as_uint(dev::div_by(a, d, y, true)) != __float_as_uint(a / d);
+    }
+    if (n) atomicAdd(bad, (unsigned long long)n);
+}
+
+// rcp_rn against IEEE division 1/d for the float bit patterns
+// [first, first + count), one pattern per thread and 16 per thread in turn.
+__global__ void __launch_bounds__(256) k_selftest_rcp(uint64_t first, uint64_t end, unsigned long long* bad) {
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < 16u; k++) {
+        const uint64_t i = first + ((uint64_t)blockIdx.x * 16u + k) * 256u + threadIdx.x;
+        if (i >= end) break;
+        const float d = __uint_as_float((uint32_t)i);
+        const float r = 1.0f / d;
+        const float y = dev::rcp_rn(d);
+        // any NaN equals any NaN (both come from the same division)
+        n += (__float_as_uint(y) != __float_as_uint(r)) & !((r != r) & (y != y));
     }
     if (n) atomicAdd(bad, (unsigned long long)n);
 }
@@ -292,6 +310,18 @@ int selftest_division(uint32_t d_begin, uint32_t d_count, unsigned long long* ba
         const uint32_t e = std::min<uint32_t>(end, j + 64u * 4096u);
         dim3 grid((1u << 23) / 256u, (e - j + 63u) / 64u);
         hipLaunchKernelGGL(k_selftest_div, grid, dim3(256), 0, s, j, e, bad_dev);
+        VRT_HIP(hipGetLastError());
+    }
+    return VRT_OK;
+}
+
+int selftest_reciprocal(uint64_t first, uint64_t count, unsigned long long* bad_dev, hipStream_t s) {
+    const uint64_t end = std::min<uint64_t>(first + count, 1ull << 32);
+    const uint64_t per_launch = 16ull * 256ull * 65536ull;  // 2^28 patterns
+    for (uint64_t j = first; j < end; j += per_launch) {
+        const uint64_t e = std::min<uint64_t>(end, j + per_launch);
+        const uint32_t blocks = (uint32_t)((e - j + 16ull * 256ull - 1) / (16ull * 256ull));
+        hipLaunchKernelGGL(k_selftest_rcp, dim3(blocks), dim3(256), 0, s, j, e, bad_dev);
         VRT_HIP(hipGetLastError());
     }
     return VRT_OK;

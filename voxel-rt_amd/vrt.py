@@ -136,6 +136,7 @@ SIGNATURES = {
     "vrt_server_status": (C.c_int, [P, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32), C.POINTER(U64)]),
     "vrt_server_clock": (C.c_double, [P]),
     "vrt_selftest_division": (C.c_int, [P, C.c_uint32, C.c_uint32, C.POINTER(U64)]),
+    "vrt_selftest_reciprocal": (C.c_int, [P, C.c_uint64, C.c_uint64, C.POINTER(U64)]),
     "vrt_world_generate": (C.c_int, [C.POINTER(vrt_world_params), P, I32]),
     "vrt_world_generate_device": (C.c_int, [P, C.POINTER(vrt_world_params)]),
     "vrt_world_surface": (C.c_int, [C.POINTER(vrt_world_params), I64, I64, C.POINTER(I32)]),
@@ -437,6 +438,11 @@ class Context:
     def selftest_division(self, d_begin, d_count):
         bad = U64(0)
         call("vrt_selftest_division", self.h, int(d_begin), int(d_count), C.byref(bad))
+        return bad.value
+
+    def selftest_reciprocal(self, first, count):
+        bad = U64(0)
+        call("vrt_selftest_reciprocal", self.h, int(first), int(count), C.byref(bad))
         return bad.value
 
     def cast_rays(self, start, direction, dist, render_dist):
