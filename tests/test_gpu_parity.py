@@ -257,13 +257,15 @@ def entering_rays(dims, n_per_face, rng):
     return np.concatenate(starts).astype(np.float32), np.concatenate(dirs).astype(np.float32)
 
 
-@pytest.mark.parametrize("dims", [(100, 37, 53), (4100, 2050, 2)], ids=["index32", "index64"])
+@pytest.mark.parametrize("dims", [(100, 37, 53), (4096, 600, 2), (4100, 2050, 2)],
+                         ids=["byte_offset32", "index32", "index64"])
 def test_cast_rays_entering_from_one_cell_outside(gpu, orc, dims):
     """A start cell at -1 (or at the dimension) on an axis, stepping into the
     grid: the voxel index of the entered cell must be exact although the start
     cell's coordinates are out of range.  X and X*Y are not multiples of 256,
     so a 24-bit unsigned multiply of -1 would give a wrong index (and a read
-    outside the grid); the second size takes the 64-bit index path."""
+    outside the grid).  The sizes take the three index forms: 32-bit byte
+    offsets, 32-bit indices (4*X*Y >= 2^23), 64-bit indices."""
     rng = np.random.default_rng(23)
     X, Y, Z = dims
     world = np.where(rng.random(X * Y * Z) < 0.08, rng.integers(0, 1 << 24, X * Y * Z), -1).astype(np.int32)

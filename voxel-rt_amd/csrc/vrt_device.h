@@ -120,13 +120,29 @@ struct Hit {
     uint32_t iters;  // loop iterations (stepCount increments)
 };
 
-// Voxel index arithmetic: 32-bit with signed 24-bit multiplies when every
-// index and partial product fits (X*Y*Z <= 2^32, X and X*Y < 2^23), else
-// 64-bit.  A start or jump cell may lie outside the grid, and one of -1 on an
-// axis can step into it, so make() must be exact modulo 2^32 for negative
-// coordinates too: the signed multiplies are, for |y|, |z| < 2^23 (cells
-// farther out cannot reach the grid in one step, and their index is never
-// read).
+// Voxel index arithmetic, three forms: IdxB keeps the byte offset of the
+// voxel (4 x index) in 32 bits when the grid is at most 2^32 bytes and 4X,
+// 4X*Y < 2^23, so the gather adds it to the grid's base address in the load
+// itself; Idx32 keeps the index in 32 bits with signed 24-bit multiplies when
+// X*Y*Z <= 2^32 and X, X*Y < 2^23; Idx64 otherwise.  A start or jump cell may
+// lie outside the grid, and one of -1 on an axis can step into it, so make()
+// must be exact modulo 2^32 for negative coordinates too: the signed
+// multiplies are, for |y|, |z| < 2^23 (cells farther out cannot reach the
+// grid in one step, and their index is never read).
+struct IdxB {
+    using T = uint32_t;  // byte offset
+    __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
+        return (uint32_t)(x * 4) + (uint32_t)__mul24(y, (int32_t)(g.X * 4)) +
+               (uint32_t)__mul24(z, (int32_t)(g.XY * 4));
+    }
+    __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
+        return 4u * (axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.X : (uint32_t)sz * (uint32_t)g.XY));
+    }
+    __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) {
+        return *(const int32_t*)((const char*)vox + i);
+    }
+    __device__ __forceinline__ static int64_t index(T i) { return (int64_t)(i >> 2); }
+};
 struct Idx32 {
     using T = uint32_t;
     __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
@@ -135,6 +151,8 @@ struct Idx32 {
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
         return axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.X : (uint32_t)sz * (uint32_t)g.XY);
     }
+    __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) { return vox[i]; }
+    __device__ __forceinline__ static int64_t index(T i) { return (int64_t)i; }
 };
 struct Idx64 {
     using T = uint64_t;
@@ -145,6 +163,8 @@ struct Idx64 {
         return axis == 0 ? (uint64_t)(int64_t)sx
                          : (axis == 1 ? (uint64_t)(int64_t)sy * g.X : (uint64_t)(int64_t)sz * g.XY);
     }
+    __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) { return vox[i]; }
+    __device__ __forceinline__ static int64_t index(T i) { return (int64_t)i; }
 };
 
 // castRay, src/fshader.glsl:59-129.  `lim` = min(float(dist), float(RENDER_DIST)):
@@ -205,7 +225,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
         idx += bx ? sdx : (by ? sdy : sdz);
         const bool inb = ((uint32_t)cx < g.X) & ((uint32_t)cy < g.Y) & ((uint32_t)cz < g.Z);
         if (!inb) break;
-        const int32_t v = vox[idx];
+        const int32_t v = IX::load(vox, idx);
         // currentIntersect: the intersect of the axis just stepped, before its update
         const float cur = bx ? ix : (by ? iy : iz);
         if (v >= 0) {
@@ -213,7 +233,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             h.pos[1] = dy * cur + sy;
             h.pos[2] = dz * cur + sz;
             axis = bx ? 0 : (by ? 1 : 2);
-            result = (int64_t)idx;
+            result = IX::index(idx);
             break;
         }
         if (v != -1) {

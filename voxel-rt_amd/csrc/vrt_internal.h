@@ -35,6 +35,8 @@ struct GridDesc {
     int32_t* top;
     // Index fits the 32-bit / 24-bit-multiply form (see Idx32).
     bool small() const { return count <= 0xffffffffull && X < (1u << 23) && XY < (1ull << 23); }
+    // Byte offset fits the 32-bit / 24-bit-multiply form (see IdxB).
+    bool bytes32() const { return count <= (1ull << 30) && 4ull * X < (1ull << 23) && 4ull * XY < (1ull << 23); }
 };
 
 // One frame as the kernels see it: vrt_frame plus derived per-frame values.

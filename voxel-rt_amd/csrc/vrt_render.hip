@@ -235,6 +235,14 @@ __global__ void k_queue_reset(unsigned int* q) {
     if (threadIdx.x < kXcds) q[threadIdx.x * kQueueStride] = 0u;
 }
 
+// Calls f with the index form the grid admits (IdxB, Idx32 or Idx64).
+template <typename F>
+void with_index(const GridDesc& g, F&& f) {
+    if (g.bytes32()) f(dev::IdxB{});
+    else if (g.small()) f(dev::Idx32{});
+    else f(dev::Idx64{});
+}
+
 }  // namespace
 
 int launch_render_plain(const GridDesc& g, const FrameDesc& f, const vrt_target& t, unsigned long long* counters,
@@ -243,12 +251,10 @@ int launch_render_plain(const GridDesc& g, const FrameDesc& f, const vrt_target&
     const int64_t blocks = (tiles + kPlainWaves - 1) / kPlainWaves;
     if (blocks <= 0) return VRT_OK;
     if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "frame too large");
-    if (g.small())
-        hipLaunchKernelGGL(k_render_plain<dev::Idx32>, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t,
+    with_index(g, [&](auto ix) {
+        hipLaunchKernelGGL(k_render_plain<decltype(ix)>, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t,
                            counters);
-    else
-        hipLaunchKernelGGL(k_render_plain<dev::Idx64>, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t,
-                           counters);
+    });
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
@@ -259,12 +265,10 @@ int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const flo
     if (n <= 0) return VRT_OK;
     const int64_t blocks = (n + 255) / 256;
     if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "too many rays");
-    if (g.small())
-        hipLaunchKernelGGL(k_cast_rays<dev::Idx32>, dim3((uint32_t)blocks), dim3(256), 0, s, g, n, start, dir, dist,
+    with_index(g, [&](auto ix) {
+        hipLaunchKernelGGL(k_cast_rays<decltype(ix)>, dim3((uint32_t)blocks), dim3(256), 0, s, g, n, start, dir, dist,
                            (float)render_dist, hit, hit_pos, face, steps);
-    else
-        hipLaunchKernelGGL(k_cast_rays<dev::Idx64>, dim3((uint32_t)blocks), dim3(256), 0, s, g, n, start, dir, dist,
-                           (float)render_dist, hit, hit_pos, face, steps);
+    });
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
@@ -284,22 +288,18 @@ int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_
         const int64_t wgs = (total_tiles + 3) / 4;
         if (wgs <= 0) return VRT_OK;
         if (wgs > 0x7fffffff) return fail(VRT_ERR_INVALID, "too many tiles in one batch");
-        if (g.small())
-            hipLaunchKernelGGL(k_render_views<dev::Idx32>, dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev,
+        with_index(g, [&](auto ix) {
+            hipLaunchKernelGGL(k_render_views<decltype(ix)>, dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev,
                                n_views, total_tiles, cfg.tile_w, cfg.tile_h, counters);
-        else
-            hipLaunchKernelGGL(k_render_views<dev::Idx64>, dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev,
-                               n_views, total_tiles, cfg.tile_w, cfg.tile_h, counters);
+        });
         VRT_HIP(hipGetLastError());
         return VRT_OK;
     }
     if (n_workgroups <= 0) return fail(VRT_ERR_INVALID, "no workgroups");
-    if (g.small())
-        hipLaunchKernelGGL(k_render_batch<dev::Idx32>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,
+    with_index(g, [&](auto ix) {
+        hipLaunchKernelGGL(k_render_batch<decltype(ix)>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,
                            n_views, total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
-    else
-        hipLaunchKernelGGL(k_render_batch<dev::Idx64>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,
-                           n_views, total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
+    });
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
