@@ -147,6 +147,35 @@ int upload_table(vrt_ctx* c) {
     c->table.dist = c->table_dist;
     c->table.n = (int32_t)packed.size();
     c->table.n_near = n_near;
+    // rows: the entries of one distance and one (dy, dz), their dx as a mask
+    std::vector<int32_t> rows;
+    std::vector<float> rows_dist;
+    int32_t n_rows_near = 0;
+    for (size_t a = 0; a < packed.size();) {
+        size_t e = a;
+        while (e < packed.size() && dist[e] == dist[a]) e++;  // one distance level
+        for (size_t i = a; i < e; i++) {
+            const int32_t dy = (packed[i] >> 5) & 31, dz = (packed[i] >> 10) & 31;
+            bool seen = false;
+            for (size_t j = a; j < i; j++) seen |= ((packed[j] >> 5) & 31) == dy && ((packed[j] >> 10) & 31) == dz;
+            if (seen) continue;
+            int32_t mask = 0;
+            for (size_t j = i; j < e; j++)
+                if (((packed[j] >> 5) & 31) == dy && ((packed[j] >> 10) & 31) == dz) mask |= 1 << ((packed[j] & 31) - 1);
+            rows.push_back(mask | (dy << 16) | (dz << 21));
+            rows_dist.push_back(dist[a]);
+            if (dist[a] > -2.0f) n_rows_near++;
+        }
+        a = e;
+    }
+    VRT_HIP(hipMalloc(&c->table_rows, rows.size() * sizeof(int32_t)));
+    VRT_HIP(hipMalloc(&c->table_rows_dist, rows_dist.size() * sizeof(float)));
+    VRT_HIP(hipMemcpy(c->table_rows, rows.data(), rows.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    VRT_HIP(hipMemcpy(c->table_rows_dist, rows_dist.data(), rows_dist.size() * sizeof(float), hipMemcpyHostToDevice));
+    c->table.rows = c->table_rows;
+    c->table.rows_dist = c->table_rows_dist;
+    c->table.n_rows = (int32_t)rows.size();
+    c->table.n_rows_near = n_rows_near;
     return VRT_OK;
 }
 
@@ -159,6 +188,8 @@ void release(vrt_ctx* c) {
     if (c->counters) hipFree(c->counters);
     if (c->table_packed) hipFree(c->table_packed);
     if (c->table_dist) hipFree(c->table_dist);
+    if (c->table_rows) hipFree(c->table_rows);
+    if (c->table_rows_dist) hipFree(c->table_rows_dist);
     if (c->bricks) hipFree(c->bricks);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;

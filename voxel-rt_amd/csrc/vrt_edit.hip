@@ -10,7 +10,8 @@
 // loops.  The table is scanned in order of decreasing distance, so the first
 // solid neighbour found is d*.  For whole-grid builds a workgroup owns one
 // 8x8x8 brick and first checks the 3x3x3 neighbouring bricks' solid flags:
-// with none solid, every empty voxel of the brick takes -6 without a scan.
+// with none solid, every empty voxel of the brick takes -6 without a scan;
+// otherwise the scan reads the neighbourhood's solidity from LDS.
 #include <hip/hip_runtime.h>
 
 #include "vrt_internal.h"
@@ -50,8 +51,19 @@ __global__ void __launch_bounds__(256) k_brick_flags(GridDesc g, uint8_t* flags,
     if (threadIdx.x == 0) flags[b] = (uint8_t)(any != 0);
 }
 
+// Whole-grid build, one workgroup per 8x8x8 brick.  A solid neighbour within
+// radius 7 lies in the 3x3x3 bricks around this one; with none, every empty
+// voxel takes -6.  Otherwise the solidity of the brick's 22x22x22
+// neighbourhood (+-7) is loaded once into LDS as one 22-bit word per x-row
+// (a ballot per row), and each voxel scans the sorted table against those
+// bits instead of gathering the grid per entry.  Writing jump words never
+// changes solidity, so the snapshot stays exact while the grid is written.
+constexpr int32_t kHalo = 7;
+constexpr int32_t kSpan = 8 + 2 * kHalo;  // 22 cells per row of the neighbourhood
+
 __global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTableDev t, const uint8_t* flags,
                                                            uint32_t bx, uint32_t by, uint32_t bz) {
+    __shared__ uint32_t rows[kSpan * kSpan];
     const uint64_t b = blockIdx.x;
     const int32_t ix = (int32_t)(b % bx), iy = (int32_t)((b / bx) % by), iz = (int32_t)(b / ((uint64_t)bx * by));
     bool near_solid = false;
@@ -64,12 +76,45 @@ __global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTable
             }
     const int32_t c = threadIdx.x;
     const int32_t x = ix * 8 + (c & 7), y = iy * 8 + ((c >> 3) & 7), z = iz * 8 + (c >> 6);
-    if ((uint32_t)x >= g.X || (uint32_t)y >= g.Y || (uint32_t)z >= g.Z) return;
+    const bool in_grid = ((uint32_t)x < g.X) & ((uint32_t)y < g.Y) & ((uint32_t)z < g.Z);
     const uint64_t i = (uint64_t)x + g.X * (uint64_t)y + g.XY * (uint64_t)z;
-    const int32_t old = g.vox[i];
-    if (old >= 0) return;
-    // A solid neighbour within radius 7 lies in the 3x3x3 bricks around this one.
-    const int32_t nv = near_solid ? fix_value(g, t, x, y, z, old) : kMinus6Bits;
+    const int32_t old = in_grid ? g.vox[i] : 0;
+    if (!near_solid) {
+        if (in_grid && old < 0 && old != kMinus6Bits) g.vox[i] = kMinus6Bits;
+        return;
+    }
+    if (!__syncthreads_or(in_grid && old < 0)) return;  // no empty voxel in the brick
+    // neighbourhood solidity: wave w fills rows 2w, 2w+1, 2w+16, ... (one row per half-wave)
+    const int32_t ox = ix * 8 - kHalo, oy = iy * 8 - kHalo, oz = iz * 8 - kHalo;
+    const int32_t lane = c & 63, half = lane >> 5, l = lane & 31;
+    for (int32_t r0 = (c >> 6) * 2; r0 < kSpan * kSpan; r0 += 16) {
+        const int32_t r = r0 + half;
+        bool solid = false;
+        if (r < kSpan * kSpan && l < kSpan) {
+            const int32_t gx = ox + l, gy = oy + r % kSpan, gz = oz + r / kSpan;
+            if (((uint32_t)gx < g.X) & ((uint32_t)gy < g.Y) & ((uint32_t)gz < g.Z))
+                solid = g.vox[(uint64_t)gx + g.X * (uint64_t)gy + g.XY * (uint64_t)gz] >= 0;
+        }
+        const uint64_t m = __ballot(solid);
+        if (l == 0 && r < kSpan * kSpan) rows[r] = (uint32_t)(m >> (half * 32)) & ((1u << kSpan) - 1u);
+    }
+    __syncthreads();
+    if (!in_grid || old >= 0) return;
+    // The table grouped by (distance, dy, dz): the first group with a solid
+    // neighbour gives d*, as the first solid entry of the sorted table would.
+    // Row word bit b is x = lx - 7 + b, so the group's dx mask (bit dx + 7)
+    // lines up with the row shifted right by lx.
+    const int32_t my_row = ((c >> 6) + kHalo) * kSpan + ((c >> 3) & 7) + kHalo;
+    const int32_t lx = c & 7;
+    int32_t nv = kMinus6Bits;
+    for (int32_t k = 0; k < t.n_rows; k++) {
+        const int32_t r = t.rows[k];
+        const int32_t dy = ((r >> 16) & 31) - 8, dz = ((r >> 21) & 31) - 8;
+        if ((rows[my_row + dz * kSpan + dy] >> lx) & (uint32_t)(r & 0x7fff)) {
+            nv = k < t.n_rows_near ? old : __float_as_int(t.rows_dist[k]);
+            break;
+        }
+    }
     if (nv != old) g.vox[i] = nv;
 }
 

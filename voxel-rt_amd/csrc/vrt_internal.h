@@ -73,10 +73,16 @@ struct ViewDesc {
 };
 
 struct DepthTableDev {
-    const int32_t* packed;  // (dx+8) | (dy+8)<<5 | (dz+8)<<10 | group<<15, sorted by dist descending
+    const int32_t* packed;  // (dx+8) | (dy+8)<<5 | (dz+8)<<10, sorted by dist descending
     const float* dist;      // dist of each packed entry
     int32_t n;              // entries with dist > -6
     int32_t n_near;         // leading entries with dist > -2
+    // The same entries grouped into rows: one per (distance, dy, dz), in the
+    // same order of distance, with the mask of their dx (bit dx+7).
+    const int32_t* rows;      // mask | (dy+8)<<16 | (dz+8)<<21
+    const float* rows_dist;   // dist of each row
+    int32_t n_rows;           // rows with dist > -6
+    int32_t n_rows_near;      // leading rows with dist > -2
 };
 
 struct Ctx;
@@ -138,6 +144,8 @@ struct vrt_ctx {
     uint64_t draws = 0;
     int32_t* table_packed = nullptr;
     float* table_dist = nullptr;
+    int32_t* table_rows = nullptr;
+    float* table_rows_dist = nullptr;
     vrt::DepthTableDev table{};
     uint8_t* bricks = nullptr;
     int32_t num_cus = 0;
