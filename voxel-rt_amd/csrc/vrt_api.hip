@@ -191,6 +191,7 @@ void release(vrt_ctx* c) {
     if (c->table_rows) hipFree(c->table_rows);
     if (c->table_rows_dist) hipFree(c->table_rows_dist);
     if (c->bricks) hipFree(c->bricks);
+    if (c->solid_bits) hipFree(c->solid_bits);
     if (c->own_stream) hipStreamDestroy(c->own_stream);
     delete c;
 }
@@ -442,14 +443,21 @@ int vrt_depth_table(float* dist, int32_t* off, int32_t cap, int32_t* count) {
 int vrt_build_depth_field(vrt_ctx* ctx) {
     if (int rc = check_ctx(ctx)) return rc;
     const GridDesc& g = ctx->grid;
-    return launch_fix_depth_box(g, ctx->table, 0, 0, 0, (int32_t)g.X, (int32_t)g.Y, (int32_t)g.Z, 0, 0, 0, -1,
-                                ctx->bricks, ctx->stream);
+    if (!ctx->solid_bits) {
+        const size_t bytes = (size_t)g.Y * g.Z * solid_bits_words(g) * sizeof(uint32_t);
+        if (hipMalloc(&ctx->solid_bits, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->solid_bits = nullptr;
+            return fail(VRT_ERR_NOMEM, "cannot allocate the depth-field build's solidity bitmap");
+        }
+    }
+    return launch_build_depth_field(g, ctx->table, ctx->bricks, ctx->solid_bits, ctx->stream);
 }
 
 // fixDepthField, src/render.cpp:226-253, over a box.
 int vrt_fix_depth_field_box(vrt_ctx* ctx, int32_t x0, int32_t y0, int32_t z0, int32_t x1, int32_t y1, int32_t z1) {
     if (int rc = check_ctx(ctx)) return rc;
-    return launch_fix_depth_box(ctx->grid, ctx->table, x0, y0, z0, x1, y1, z1, 0, 0, 0, -1, nullptr, ctx->stream);
+    return launch_fix_depth_box(ctx->grid, ctx->table, x0, y0, z0, x1, y1, z1, 0, 0, 0, -1, ctx->stream);
 }
 
 // removeSphere, src/level.cpp:30-56.
@@ -463,7 +471,7 @@ int vrt_remove_sphere(vrt_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t 
                   z0 = (int64_t)cz - R, z1 = (int64_t)cz + R;
     auto c32 = [](int64_t v) { return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v)); };
     return launch_fix_depth_box(ctx->grid, ctx->table, c32(x0), c32(y0), c32(z0), c32(x1), c32(y1), c32(z1), cx, cy,
-                                cz, R, nullptr, ctx->stream);
+                                cz, R, ctx->stream);
 }
 
 // placeVoxel, src/render.cpp:256-262.

@@ -37,18 +37,40 @@ __device__ __forceinline__ int32_t fix_value(const GridDesc& g, const DepthTable
     return kMinus6Bits;
 }
 
-// Any solid voxel in each 8x8x8 brick.
-__global__ void __launch_bounds__(256) k_brick_flags(GridDesc g, uint8_t* flags, uint32_t bx, uint32_t by,
-                                                      uint32_t bz) {
-    const uint64_t b = blockIdx.x;
+// Solidity bitmap of the grid: row (y, z) is W words, bit x + 32 of the row
+// is voxel x (bits of x < 0 and x >= X are 0), so any 22-bit window starting
+// at x >= -7 lies in two consecutive words.  One wave per 64 voxels of a row.
+__global__ void __launch_bounds__(256) k_solid_bits(GridDesc g, uint32_t* bits, uint32_t W, uint32_t chunks) {
+    const uint64_t wave = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t row = wave / chunks;  // y + Y * z
+    if (row >= (uint64_t)g.Y * g.Z) return;
+    const uint32_t j = (uint32_t)(wave % chunks);
+    const uint32_t x = j * 64 + lane, y = (uint32_t)(row % g.Y), z = (uint32_t)(row / g.Y);
+    bool solid = false;
+    if (x < g.X) solid = g.vox[(uint64_t)x + g.X * (uint64_t)y + g.XY * (uint64_t)z] >= 0;
+    const uint64_t m = __ballot(solid);
+    uint32_t* r = bits + row * W;
+    if (lane == 0) r[1 + 2 * j] = (uint32_t)m;
+    if (lane == 32) r[2 + 2 * j] = (uint32_t)(m >> 32);
+}
+
+// Any solid voxel in each 8x8x8 brick, from the bitmap: one wave per brick,
+// one lane per (y, z) row of the brick, 8 bits each.
+__global__ void __launch_bounds__(256) k_brick_flags_bits(GridDesc g, const uint32_t* bits, uint32_t W,
+                                                           uint8_t* flags, uint32_t bx, uint32_t by, uint64_t nb) {
+    const uint64_t b = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= nb) return;
+    const uint32_t lane = threadIdx.x & 63;
     const uint32_t ix = (uint32_t)(b % bx), iy = (uint32_t)((b / bx) % by), iz = (uint32_t)(b / ((uint64_t)bx * by));
-    int any = 0;
-    for (int c = threadIdx.x; c < 512; c += 256) {
-        const uint32_t x = ix * 8 + (c & 7), y = iy * 8 + ((c >> 3) & 7), z = iz * 8 + (c >> 6);
-        if (x < g.X && y < g.Y && z < g.Z) any |= g.vox[(uint64_t)x + g.X * (uint64_t)y + g.XY * z] >= 0;
+    const uint32_t y = iy * 8 + (lane & 7), z = iz * 8 + (lane >> 3);
+    uint32_t v = 0;
+    if (y < g.Y && z < g.Z) {
+        const uint32_t bit = ix * 8 + 32;
+        v = (bits[((uint64_t)y + (uint64_t)g.Y * z) * W + (bit >> 5)] >> (bit & 31)) & 0xffu;
     }
-    any = __syncthreads_or(any);
-    if (threadIdx.x == 0) flags[b] = (uint8_t)(any != 0);
+    const bool any = __ballot(v != 0u) != 0;
+    if (lane == 0) flags[b] = (uint8_t)any;
 }
 
 // Whole-grid build, one workgroup per 8x8x8 brick.  A solid neighbour within
@@ -62,7 +84,8 @@ constexpr int32_t kHalo = 7;
 constexpr int32_t kSpan = 8 + 2 * kHalo;  // 22 cells per row of the neighbourhood
 
 __global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTableDev t, const uint8_t* flags,
-                                                           uint32_t bx, uint32_t by, uint32_t bz) {
+                                                           const uint32_t* bits, uint32_t W, uint32_t bx,
+                                                           uint32_t by, uint32_t bz) {
     __shared__ uint32_t rows[kSpan * kSpan];
     const uint64_t b = blockIdx.x;
     const int32_t ix = (int32_t)(b % bx), iy = (int32_t)((b / bx) % by), iz = (int32_t)(b / ((uint64_t)bx * by));
@@ -84,19 +107,19 @@ __global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTable
         return;
     }
     if (!__syncthreads_or(in_grid && old < 0)) return;  // no empty voxel in the brick
-    // neighbourhood solidity: wave w fills rows 2w, 2w+1, 2w+16, ... (one row per half-wave)
+    // neighbourhood solidity: thread r takes row r, the 22 bits from x = ox
+    // (bit ox + 32 >= 25 of the bitmap row) out of two words
     const int32_t ox = ix * 8 - kHalo, oy = iy * 8 - kHalo, oz = iz * 8 - kHalo;
-    const int32_t lane = c & 63, half = lane >> 5, l = lane & 31;
-    for (int32_t r0 = (c >> 6) * 2; r0 < kSpan * kSpan; r0 += 16) {
-        const int32_t r = r0 + half;
-        bool solid = false;
-        if (r < kSpan * kSpan && l < kSpan) {
-            const int32_t gx = ox + l, gy = oy + r % kSpan, gz = oz + r / kSpan;
-            if (((uint32_t)gx < g.X) & ((uint32_t)gy < g.Y) & ((uint32_t)gz < g.Z))
-                solid = g.vox[(uint64_t)gx + g.X * (uint64_t)gy + g.XY * (uint64_t)gz] >= 0;
+    if (c < kSpan * kSpan) {
+        const int32_t gy = oy + c % kSpan, gz = oz + c / kSpan;
+        uint32_t v = 0;
+        if (((uint32_t)gy < g.Y) & ((uint32_t)gz < g.Z)) {
+            const uint32_t* rw = bits + ((uint64_t)gy + (uint64_t)g.Y * (uint64_t)gz) * W;
+            const uint32_t b0 = (uint32_t)(ox + 32);
+            const uint64_t two = (uint64_t)rw[b0 >> 5] | ((uint64_t)rw[(b0 >> 5) + 1] << 32);
+            v = (uint32_t)(two >> (b0 & 31)) & ((1u << kSpan) - 1u);
         }
-        const uint64_t m = __ballot(solid);
-        if (l == 0 && r < kSpan * kSpan) rows[r] = (uint32_t)(m >> (half * 32)) & ((1u << kSpan) - 1u);
+        rows[c] = v;
     }
     __syncthreads();
     if (!in_grid || old >= 0) return;
@@ -107,14 +130,29 @@ __global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTable
     const int32_t my_row = ((c >> 6) + kHalo) * kSpan + ((c >> 3) & 7) + kHalo;
     const int32_t lx = c & 7;
     int32_t nv = kMinus6Bits;
-    for (int32_t k = 0; k < t.n_rows; k++) {
-        const int32_t r = t.rows[k];
+    // four groups per step: their LDS reads are in flight together
+    auto hits = [&](int32_t r) {
         const int32_t dy = ((r >> 16) & 31) - 8, dz = ((r >> 21) & 31) - 8;
-        if ((rows[my_row + dz * kSpan + dy] >> lx) & (uint32_t)(r & 0x7fff)) {
-            nv = k < t.n_rows_near ? old : __float_as_int(t.rows_dist[k]);
+        return ((rows[my_row + dz * kSpan + dy] >> lx) & (uint32_t)(r & 0x7fff)) != 0u;
+    };
+    int32_t found = -1;
+    int32_t k = 0;
+    for (; k + 8 <= t.n_rows; k += 8) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) m |= (uint32_t)hits(t.rows[k + j]) << j;
+        if (m) {
+            found = k + __builtin_ctz(m);
             break;
         }
     }
+    if (found < 0)
+        for (; k < t.n_rows; k++)
+            if (hits(t.rows[k])) {
+                found = k;
+                break;
+            }
+    if (found >= 0) nv = found < t.n_rows_near ? old : __float_as_int(t.rows_dist[found]);
     if (nv != old) g.vox[i] = nv;
 }
 
@@ -208,31 +246,40 @@ inline void clip(int64_t& lo, int64_t& hi, int64_t n) {
 }  // namespace
 
 int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, int32_t y0, int32_t z0, int32_t x1,
-                         int32_t y1, int32_t z1, int32_t scx, int32_t scy, int32_t scz, int32_t sr,
-                         uint8_t* brick_scratch, hipStream_t s) {
+                         int32_t y1, int32_t z1, int32_t scx, int32_t scy, int32_t scz, int32_t sr, hipStream_t s) {
     int64_t ax = x0, bx_ = x1, ay = y0, by_ = y1, az = z0, bz_ = z1;
     clip(ax, bx_, g.X);
     clip(ay, by_, g.Y);
     clip(az, bz_, g.Z);
     if (bx_ <= ax || by_ <= ay || bz_ <= az) return VRT_OK;
-    const bool whole = ax == 0 && ay == 0 && az == 0 && bx_ == g.X && by_ == g.Y && bz_ == g.Z && sr < 0;
-    if (whole && brick_scratch) {
-        const uint32_t nbx = (g.X + 7) / 8, nby = (g.Y + 7) / 8, nbz = (g.Z + 7) / 8;
-        const uint64_t nb = (uint64_t)nbx * nby * nbz;
-        if (nb > 0x7fffffffull) return fail(VRT_ERR_INVALID, "grid too large for brick build");
-        hipLaunchKernelGGL(k_brick_flags, dim3((uint32_t)nb), dim3(256), 0, s, g, brick_scratch, nbx, nby, nbz);
-        VRT_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_fix_depth_bricks, dim3((uint32_t)nb), dim3(512), 0, s, g, t, brick_scratch, nbx, nby,
-                           nbz);
-        VRT_HIP(hipGetLastError());
-        return VRT_OK;
-    }
     const int64_t w = bx_ - ax, h = by_ - ay, d = bz_ - az;
     const int64_t n = w * h * d;
     const int64_t blocks = (n + 255) / 256;
     if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "box too large");
     hipLaunchKernelGGL(k_fix_depth_box, dim3((uint32_t)blocks), dim3(256), 0, s, g, t, (int32_t)ax, (int32_t)ay,
                        (int32_t)az, (int32_t)w, (int32_t)h, (int32_t)d, scx, scy, scz, sr);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+uint32_t solid_bits_words(const GridDesc& g) { return (g.X + 64u + 31u) / 32u; }
+
+int launch_build_depth_field(const GridDesc& g, const DepthTableDev& t, uint8_t* bricks, uint32_t* bits,
+                             hipStream_t s) {
+    const uint32_t W = solid_bits_words(g);
+    const uint64_t rows = (uint64_t)g.Y * g.Z;
+    VRT_HIP(hipMemsetAsync(bits, 0, rows * W * sizeof(uint32_t), s));
+    const uint32_t chunks = (g.X + 63u) / 64u;
+    const uint64_t bit_blocks = (rows * chunks + 3) / 4;
+    const uint32_t nbx = (g.X + 7) / 8, nby = (g.Y + 7) / 8, nbz = (g.Z + 7) / 8;
+    const uint64_t nb = (uint64_t)nbx * nby * nbz;
+    if (bit_blocks > 0x7fffffffull || nb > 0x7fffffffull) return fail(VRT_ERR_INVALID, "grid too large for the build");
+    hipLaunchKernelGGL(k_solid_bits, dim3((uint32_t)bit_blocks), dim3(256), 0, s, g, bits, W, chunks);
+    VRT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_brick_flags_bits, dim3((uint32_t)((nb + 3) / 4)), dim3(256), 0, s, g, bits, W, bricks, nbx,
+                       nby, nb);
+    VRT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fix_depth_bricks, dim3((uint32_t)nb), dim3(512), 0, s, g, t, bricks, bits, W, nbx, nby, nbz);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }

@@ -103,7 +103,12 @@ int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu);
 
 int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, int32_t y0, int32_t z0,
                          int32_t x1, int32_t y1, int32_t z1, int32_t sphere_cx, int32_t sphere_cy,
-                         int32_t sphere_cz, int32_t sphere_r, uint8_t* brick_scratch, hipStream_t s);
+                         int32_t sphere_cz, int32_t sphere_r, hipStream_t s);
+// Whole-grid depth field: solidity bitmap (Y*Z rows of solid_bits_words(g)
+// words), brick flags (one byte per 8x8x8 brick), then the brick kernel.
+uint32_t solid_bits_words(const GridDesc& g);
+int launch_build_depth_field(const GridDesc& g, const DepthTableDev& t, uint8_t* bricks, uint32_t* bits,
+                             hipStream_t s);
 int launch_carve_sphere(const GridDesc& g, int32_t cx, int32_t cy, int32_t cz, int32_t r, hipStream_t s);
 int launch_set_voxel(const GridDesc& g, int64_t index, int32_t y, int32_t v, hipStream_t s);
 // Raises *g.top to the highest y with a solid voxel in the box [x0,x1) x
@@ -148,6 +153,7 @@ struct vrt_ctx {
     float* table_rows_dist = nullptr;
     vrt::DepthTableDev table{};
     uint8_t* bricks = nullptr;
+    uint32_t* solid_bits = nullptr;  // whole-grid build scratch, allocated by the first build
     int32_t num_cus = 0;
     bool tuned = false;
     vrt_batch_config tuned_cfg{8, 8, 0, 0};
