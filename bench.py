@@ -50,13 +50,16 @@ def load():
 def profiled(key):
     """A per-launch figure of k_render_plain measured with rocprofv3 --pmc, from
     the committed counter summary: "traffic_bytes_per_launch" (calibrated
-    FETCH_SIZE + WRITE_SIZE) or "valu_lane_ops_per_launch"
-    (SQ_THREAD_CYCLES_VALU, one per active lane per VALU instruction)."""
+    FETCH_SIZE + WRITE_SIZE), "valu_lane_ops_per_launch"
+    (SQ_THREAD_CYCLES_VALU, one per active lane per VALU instruction) or
+    "valu_issue_frac" (4 x SQ_INSTS_VALU over 1024 SIMDs x the launch's
+    cycles: the share of VALU issue slots used)."""
     path = os.path.join(ROOT, "profiles", "k_render_plain_c2_1080p_counters.md")
     try:
         for line in open(path):
             if line.startswith(key + ":"):
-                return int(line.split(":")[1])
+                v = line.split(":")[1]
+                return float(v) if "." in v else int(v)
     except OSError:
         pass
     return None
@@ -119,7 +122,8 @@ def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
     if lane_ops:
         valu_achieved = lane_ops / (kernel_ms * 1e-3) / 1e12
         valu = {"achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "T lane-ops/s",
-                "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops}
+                "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops,
+                "issue_frac_profiled": profiled("valu_issue_frac")}
     # The same frames with the shadow cutoff off: every output is identical,
     # the DDA walks are the reference's full length (its iteration count).
     full = None
@@ -362,15 +366,18 @@ def cpu_baseline(vrt, sc, ctx):
     wdf = ctx.download()
     n = orc.threads()
     orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height, nthreads=n)  # warm-up
+    # whole frames until about 10 s of wall time (at least 3)
     times, rays = [], 0
-    for _ in range(3):
+    t_start = time.perf_counter()
+    while len(times) < 3 or time.perf_counter() - t_start < 10.0:
         t = time.perf_counter()
         r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height, nthreads=n)
         times.append(time.perf_counter() - t)
         rays = r["rays"]
     med = float(np.median(times))
     return {"value": rays / med / 1e6, "unit": "Mrays/s", "cores": n, "kind": "port",
-            "sample": "full 1920x1080 frame of the same scene, median of 3 after 1 warm-up (%.3f s/frame)" % med}
+            "sample": "full 1920x1080 frames of the same scene for ~10 s after 1 warm-up: median of %d "
+                      "(%.3f s/frame)" % (len(times), med)}
 
 
 def main():
