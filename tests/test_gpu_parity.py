@@ -240,6 +240,28 @@ def test_c2_frame_matches_oracle(gpu, orc):
     ctx.close()
 
 
+def test_c2_more_cameras_match_oracle(gpu, orc):
+    """Six more 1080p cameras over the C2 world (lights clustered around the
+    camera so that many are in range, one depth-view frame), cutoff on."""
+    sc = gpu.make_scene("c2", seed=1)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    rng = np.random.default_rng(31)
+    for k in range(6):
+        f = gpu.make_frame(sc.params, sc.dims, sc.world, rng, 1920, 1080, 16, 384, light_spread=48.0,
+                           depth_view=1 if k == 5 else 0)
+        t = gpu.Target(1920, 1080, depth=True, aux=True)
+        ctx.stats(reset=True)
+        ctx.draw(t, frame=f)
+        g = t.download()
+        g.update(ctx.stats(reset=True))
+        r = orc.render(wdf, sc.dims, f, 1920, 1080)
+        assert_frames_match(g, r, "c2 camera %d" % k)
+    ctx.close()
+
+
 def entering_rays(dims, n_per_face, rng):
     """Rays whose start cell is one cell outside the grid on one axis (-1 or
     the dimension) and whose direction points into the grid on that axis."""
