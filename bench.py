@@ -307,6 +307,17 @@ def served_bench(vrt, n_requests=240, load=0.7):
     wall = vrt.lib().vrt_server_clock(s) - t_start
     st = ctx.stats(reset=True)
     vrt.call("vrt_server_destroy", s)
+    # one removeSphere (carve + depth-field repair) on its own, device time
+    # between events on the context's stream, radius 1-8 at surface points
+    edit_us = []
+    for _ in range(40):
+        x, z = int(rng.integers(8, X - 8)), int(rng.integers(8, Z - 8))
+        y = vrt.surface(params, x, z)
+        e0.record(ctx.stream)
+        ctx.remove_sphere(x, y, z, int(rng.integers(1, 9)))
+        e1.record(ctx.stream)
+        ctx.sync()
+        edit_us.append(e1.ms_since(e0) * 1e3)
     ctx.close()
     return {
         "workload": "resident 1024x512x1024 grid (2 GiB), %d Poisson requests at %.0f req/s (%.0f%% of the plain "
@@ -314,6 +325,8 @@ def served_bench(vrt, n_requests=240, load=0.7):
                     % (n_requests, rate, 100 * load, edits),
         "latency_ms": percentiles(lat), "mrays_s": st["rays"] / wall / 1e6, "rays": int(st["rays"]),
         "wall_s": wall, "plain_ms_per_request": mean_ms,
+        "remove_sphere_us": {"p50": float(np.percentile(edit_us, 50)), "p99": float(np.percentile(edit_us, 99)),
+                             "sample": "40 edits, radius 1-8 at surface points, carve + repair"},
     }
 
 

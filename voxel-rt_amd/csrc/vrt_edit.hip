@@ -22,21 +22,6 @@ namespace {
 
 constexpr int32_t kMinus6Bits = (int32_t)0xC0C00000;  // bits of -6.0f
 
-__device__ __forceinline__ int32_t fix_value(const GridDesc& g, const DepthTableDev& t, int32_t x, int32_t y,
-                                             int32_t z, int32_t old) {
-    for (int32_t k = 0; k < t.n; k++) {
-        const int32_t p = t.packed[k];
-        const int32_t nx = x + (p & 31) - 8;
-        const int32_t ny = y + ((p >> 5) & 31) - 8;
-        const int32_t nz = z + ((p >> 10) & 31) - 8;
-        if (((uint32_t)nx < g.X) & ((uint32_t)ny < g.Y) & ((uint32_t)nz < g.Z)) {
-            const uint64_t i = (uint64_t)nx + (uint64_t)g.X * (uint64_t)ny + g.XY * (uint64_t)nz;
-            if (g.vox[i] >= 0) return k < t.n_near ? old : __float_as_int(t.dist[k]);
-        }
-    }
-    return kMinus6Bits;
-}
-
 // Solidity bitmap of the grid: row (y, z) is W words, bit x + 32 of the row
 // is voxel x (bits of x < 0 and x >= X are 0), so any 22-bit window starting
 // at x >= -7 lies in two consecutive words.  One wave per 64 voxels of a row.
@@ -83,6 +68,60 @@ __global__ void __launch_bounds__(256) k_brick_flags_bits(GridDesc g, const uint
 constexpr int32_t kHalo = 7;
 constexpr int32_t kSpan = 8 + 2 * kHalo;  // 22 cells per row of the neighbourhood
 
+// fixDepthField's result for the voxel at (lx, ly, lz) of a brick whose
+// 22x22x22 neighbourhood solidity is in `rows` (bit b of row (ry, rz) is
+// x = lx - 7 + b).  The table is grouped by (distance, dy, dz): the first
+// group with a solid neighbour gives d*, as the first solid entry of the
+// sorted table would, and the group's dx mask (bit dx + 7) lines up with the
+// row shifted right by lx.  Eight groups per step keep eight LDS reads in
+// flight.
+__device__ __forceinline__ int32_t scan_groups(const uint32_t* rows, const DepthTableDev& t, int32_t lx, int32_t ly,
+                                               int32_t lz, int32_t old) {
+    const int32_t my_row = (lz + kHalo) * kSpan + ly + kHalo;
+    auto hits = [&](int32_t r) {
+        const int32_t dy = ((r >> 16) & 31) - 8, dz = ((r >> 21) & 31) - 8;
+        return ((rows[my_row + dz * kSpan + dy] >> lx) & (uint32_t)(r & 0x7fff)) != 0u;
+    };
+    int32_t found = -1;
+    int32_t k = 0;
+    for (; k + 8 <= t.n_rows; k += 8) {
+        uint32_t m = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) m |= (uint32_t)hits(t.rows[k + j]) << j;
+        if (m) {
+            found = k + __builtin_ctz(m);
+            break;
+        }
+    }
+    if (found < 0)
+        for (; k < t.n_rows; k++)
+            if (hits(t.rows[k])) {
+                found = k;
+                break;
+            }
+    if (found < 0) return kMinus6Bits;
+    return found < t.n_rows_near ? old : __float_as_int(t.rows_dist[found]);
+}
+
+// The neighbourhood solidity of the brick at (bx0, by0, bz0) read from the
+// grid itself: thread r < 484 gathers the 22 voxels of row r (independent
+// loads, all in flight) into one word.
+__device__ __forceinline__ void fill_rows_from_grid(const GridDesc& g, uint32_t* rows, int32_t bx0, int32_t by0,
+                                                    int32_t bz0, int32_t c) {
+    if (c >= kSpan * kSpan) return;
+    const int32_t gy = by0 - kHalo + c % kSpan, gz = bz0 - kHalo + c / kSpan;
+    uint32_t v = 0;
+    if (((uint32_t)gy < g.Y) & ((uint32_t)gz < g.Z)) {
+        const int32_t* row = g.vox + (uint64_t)g.X * (uint64_t)gy + g.XY * (uint64_t)gz;
+#pragma unroll
+        for (int32_t b = 0; b < kSpan; b++) {
+            const int32_t gx = bx0 - kHalo + b;
+            if ((uint32_t)gx < g.X && row[gx] >= 0) v |= 1u << b;
+        }
+    }
+    rows[c] = v;
+}
+
 __global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTableDev t, const uint8_t* flags,
                                                            const uint32_t* bits, uint32_t W, uint32_t bx,
                                                            uint32_t by, uint32_t bz) {
@@ -123,57 +162,36 @@ __global__ void __launch_bounds__(512) k_fix_depth_bricks(GridDesc g, DepthTable
     }
     __syncthreads();
     if (!in_grid || old >= 0) return;
-    // The table grouped by (distance, dy, dz): the first group with a solid
-    // neighbour gives d*, as the first solid entry of the sorted table would.
-    // Row word bit b is x = lx - 7 + b, so the group's dx mask (bit dx + 7)
-    // lines up with the row shifted right by lx.
-    const int32_t my_row = ((c >> 6) + kHalo) * kSpan + ((c >> 3) & 7) + kHalo;
-    const int32_t lx = c & 7;
-    int32_t nv = kMinus6Bits;
-    // four groups per step: their LDS reads are in flight together
-    auto hits = [&](int32_t r) {
-        const int32_t dy = ((r >> 16) & 31) - 8, dz = ((r >> 21) & 31) - 8;
-        return ((rows[my_row + dz * kSpan + dy] >> lx) & (uint32_t)(r & 0x7fff)) != 0u;
-    };
-    int32_t found = -1;
-    int32_t k = 0;
-    for (; k + 8 <= t.n_rows; k += 8) {
-        uint32_t m = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) m |= (uint32_t)hits(t.rows[k + j]) << j;
-        if (m) {
-            found = k + __builtin_ctz(m);
-            break;
-        }
-    }
-    if (found < 0)
-        for (; k < t.n_rows; k++)
-            if (hits(t.rows[k])) {
-                found = k;
-                break;
-            }
-    if (found >= 0) nv = found < t.n_rows_near ? old : __float_as_int(t.rows_dist[found]);
+    const int32_t nv = scan_groups(rows, t, c & 7, (c >> 3) & 7, c >> 6, old);
     if (nv != old) g.vox[i] = nv;
 }
 
 // fixDepthField over the box [x0,x1) x [y0,y1) x [z0,z1) (already clipped);
-// with sr >= 0 only voxels with (x-cx)^2+(y-cy)^2+(z-cz)^2 < sr^2.
-__global__ void __launch_bounds__(256) k_fix_depth_box(GridDesc g, DepthTableDev t, int32_t x0, int32_t y0,
-                                                        int32_t z0, int32_t w, int32_t h, int32_t d, int32_t cx,
-                                                        int32_t cy, int32_t cz, int32_t sr) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (int64_t)w * h * d) return;
-    const int32_t x = x0 + (int32_t)(k % w);
-    const int32_t y = y0 + (int32_t)((k / w) % h);
-    const int32_t z = z0 + (int32_t)(k / ((int64_t)w * h));
+// with sr >= 0 only voxels with (x-cx)^2+(y-cy)^2+(z-cz)^2 < sr^2.  The box
+// is tiled into 8x8x8 bricks from its corner, one workgroup each: the
+// brick's neighbourhood solidity is gathered from the grid into LDS and its
+// voxels scan the grouped table against it.
+__global__ void __launch_bounds__(512) k_fix_depth_box(GridDesc g, DepthTableDev t, int32_t x0, int32_t y0,
+                                                        int32_t z0, int32_t w, int32_t h, int32_t d, int32_t nbx,
+                                                        int32_t nby, int32_t cx, int32_t cy, int32_t cz, int32_t sr) {
+    __shared__ uint32_t rows[kSpan * kSpan];
+    const int32_t b = (int32_t)blockIdx.x;
+    const int32_t bx0 = x0 + (b % nbx) * 8, by0 = y0 + ((b / nbx) % nby) * 8, bz0 = z0 + (b / (nbx * nby)) * 8;
+    const int32_t c = threadIdx.x;
+    const int32_t lx = c & 7, ly = (c >> 3) & 7, lz = c >> 6;
+    const int32_t x = bx0 + lx, y = by0 + ly, z = bz0 + lz;
+    bool mine = (x < x0 + w) & (y < y0 + h) & (z < z0 + d);
     if (sr >= 0) {
         const int32_t ex = x - cx, ey = y - cy, ez = z - cz;
-        if (ex * ex + ey * ey + ez * ez >= sr * sr) return;
+        mine &= ex * ex + ey * ey + ez * ez < sr * sr;
     }
     const uint64_t i = (uint64_t)x + g.X * (uint64_t)y + g.XY * (uint64_t)z;
-    const int32_t old = g.vox[i];
-    if (old >= 0) return;
-    const int32_t nv = fix_value(g, t, x, y, z, old);
+    const int32_t old = mine ? g.vox[i] : 0;
+    if (!__syncthreads_or(mine && old < 0)) return;  // nothing to fix in this brick
+    fill_rows_from_grid(g, rows, bx0, by0, bz0, c);
+    __syncthreads();
+    if (!mine || old >= 0) return;
+    const int32_t nv = scan_groups(rows, t, lx, ly, lz, old);
     if (nv != old) g.vox[i] = nv;
 }
 
@@ -253,11 +271,12 @@ int launch_fix_depth_box(const GridDesc& g, const DepthTableDev& t, int32_t x0, 
     clip(az, bz_, g.Z);
     if (bx_ <= ax || by_ <= ay || bz_ <= az) return VRT_OK;
     const int64_t w = bx_ - ax, h = by_ - ay, d = bz_ - az;
-    const int64_t n = w * h * d;
-    const int64_t blocks = (n + 255) / 256;
+    const int64_t nbx = (w + 7) / 8, nby = (h + 7) / 8, nbz = (d + 7) / 8;
+    const int64_t blocks = nbx * nby * nbz;
     if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "box too large");
-    hipLaunchKernelGGL(k_fix_depth_box, dim3((uint32_t)blocks), dim3(256), 0, s, g, t, (int32_t)ax, (int32_t)ay,
-                       (int32_t)az, (int32_t)w, (int32_t)h, (int32_t)d, scx, scy, scz, sr);
+    hipLaunchKernelGGL(k_fix_depth_box, dim3((uint32_t)blocks), dim3(512), 0, s, g, t, (int32_t)ax, (int32_t)ay,
+                       (int32_t)az, (int32_t)w, (int32_t)h, (int32_t)d, (int32_t)nbx, (int32_t)nby, scx, scy, scz,
+                       sr);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
