@@ -125,12 +125,14 @@ int vrt_grid_device_ptr(vrt_ctx* ctx, int32_t** out);
  * vrt_grid_device_ptr.  The library's own uploads and edits keep it current.
  * No counterpart in the reference, whose grid has no device-side writers. */
 int vrt_grid_modified(vrt_ctx* ctx);
-/* Shadow cutoff (on by default): a shadow ray that is above every solid voxel
- * of the grid and not heading down stops there as a miss instead of walking on
- * to the grid's edge or its budget.  Every output equals the reference's
- * either way; only vrt_stats.dda_steps (iterations executed) differs: with the
- * cutoff off it equals the reference's count.  Applies to later draws and to
- * batches created later.  No counterpart in the reference. */
+/* Cutoff (on by default): a shadow ray that is above every solid voxel of the
+ * grid and not heading down stops there as a miss instead of walking on to the
+ * grid's edge or its budget; so does a primary ray when nothing shows its step
+ * count (the frame is not the depth view and the target has no step map).
+ * Every output equals the reference's either way; only vrt_stats.dda_steps
+ * (iterations executed) differs: with the cutoff off it equals the
+ * reference's count.  Applies to later draws and to batches created later.
+ * No counterpart in the reference. */
 int vrt_set_shadow_cutoff(vrt_ctx* ctx, int32_t enable);
 int vrt_grid_dims(vrt_ctx* ctx, int64_t dims[3]);
 

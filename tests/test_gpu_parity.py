@@ -22,6 +22,8 @@ def same_floats(a, b):
 
 def assert_frames_match(g, r, label=""):
     for k in ("hit", "face", "shadow_eval", "shadow_occ", "steps"):
+        if k not in g:  # a target without that map
+            continue
         bad = np.argwhere(g[k] != r[k])
         assert bad.size == 0, "%s %s differs at %d pixels, first %s: gpu %s oracle %s" % (
             label, k, len(bad), bad[0].tolist(), g[k][tuple(bad[0])], r[k][tuple(bad[0])])
@@ -259,6 +261,50 @@ def test_c2_more_cameras_match_oracle(gpu, orc):
         g.update(ctx.stats(reset=True))
         r = orc.render(wdf, sc.dims, f, 1920, 1080)
         assert_frames_match(g, r, "c2 camera %d" % k)
+    ctx.close()
+
+
+@pytest.mark.parametrize("case", ["c2_cameras", "c1"])
+def test_primary_cutoff_keeps_every_output(gpu, orc, case):
+    """Without a step map (and outside the depth view) primary rays heading
+    up stop once above every solid voxel, like shadow rays: every map the
+    target has still equals the oracle's, the iteration total drops below the
+    oracle's, and with a step map (or the cutoff off) nothing stops early."""
+    sc = gpu.make_scene("c2" if case == "c2_cameras" else "c1", seed=1 if case == "c2_cameras" else 4)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    rng = np.random.default_rng(77)
+    frames = [sc.frame]
+    for _ in range(2):
+        # pitched up as well as down, so that many primary rays end in the sky
+        f = gpu.make_frame(sc.params, sc.dims, sc.world, rng, sc.width, sc.height, 16, 384,
+                           pitch_range=(-0.4, 0.3), light_spread=48.0)
+        frames.append(f)
+    for k, f in enumerate(frames):
+        r = orc.render(wdf, sc.dims, f, sc.width, sc.height)
+        res = {}
+        for steps in (False, True):
+            t = gpu.Target(sc.width, sc.height, depth=True, aux=True, steps=steps)
+            ctx.stats(reset=True)
+            ctx.draw(t, frame=f)
+            g = t.download()
+            g.update(ctx.stats(reset=True))
+            assert_frames_match(g, r, "%s frame %d steps=%d" % (case, k, steps))
+            res[steps] = g["dda_steps"]
+        ctx.set_shadow_cutoff(False)
+        t = gpu.Target(sc.width, sc.height, depth=True, aux=True, steps=False)
+        ctx.stats(reset=True)
+        ctx.draw(t, frame=f)
+        g = t.download()
+        g.update(ctx.stats(reset=True))
+        g["shadow_cutoff"] = False
+        assert_frames_match(g, r, "%s frame %d cutoff off" % (case, k))
+        ctx.set_shadow_cutoff(True)
+        assert res[False] <= res[True] <= r["dda_steps"]
+        if k > 0:
+            assert res[False] < res[True], "no primary ray stopped early in frame %d" % k
     ctx.close()
 
 

@@ -86,13 +86,15 @@ float sqrt_le_limit(float r) {
     return t;
 }
 
-FrameDesc make_frame_desc(const vrt_frame& f, int32_t width, int32_t height) {
+FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target) {
+    const int32_t width = target.width, height = target.height;
     FrameDesc d;
     d.f = f;
     d.render_lim = (float)f.render_dist;
     d.light_valid = 0;
     d.inv_w = width > 0 ? 1.0f / (float)width : 0.0f;
     d.inv_h = height > 0 ? 1.0f / (float)height : 0.0f;
+    d.primary_cutoff = (f.view_depth_field != 1 && target.steps == nullptr) ? 1 : 0;
     for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
         const vrt_local_light& L = f.lights[i];
         d.light_dd_max[i] = sqrt_le_limit(L.radius);
@@ -585,7 +587,7 @@ int vrt_draw_frame(vrt_ctx* ctx, const vrt_frame* f, const vrt_target* t) {
     if (int rc = check_frame(*f)) return rc;
     if (int rc = check_target(*t)) return rc;
     ctx->draws++;
-    return launch_render_plain(render_grid(ctx), make_frame_desc(*f, t->width, t->height), *t, ctx->counters, ctx->stream);
+    return launch_render_plain(render_grid(ctx), make_frame_desc(*f, *t), *t, ctx->counters, ctx->stream);
 }
 
 // glDrawArrays over the fullscreen quad, src/main.cpp:59.

@@ -74,7 +74,7 @@ int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& c
         if (int rc = check_target(views[i].target)) return rc;
         ViewDesc d;
         std::memset(&d, 0, sizeof(d));
-        d.frame = make_frame_desc(views[i].frame, views[i].target.width, views[i].target.height);
+        d.frame = make_frame_desc(views[i].frame, views[i].target);
         d.target = views[i].target;
         d.tiles_x = (views[i].target.width + cfg.tile_w - 1) / cfg.tile_w;
         const int32_t tiles_y = (views[i].target.height + cfg.tile_h - 1) / cfg.tile_h;

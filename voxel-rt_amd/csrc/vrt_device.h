@@ -174,8 +174,8 @@ struct Idx64 {
 // between two in-grid cells every step was checked, so it never wrapped.
 // The axis choice is computed with selects so the wave does not branch.
 //
-// ytop (shadow rays): an upper bound on the y of every solid voxel, or
-// INT32_MAX.  Let dy >= 0.  Every cell of a segment has y >= int(start.y),
+// ytop (shadow rays, and primary rays whose step count nothing shows): an
+// upper bound on the y of every solid voxel, or INT32_MAX.  Let dy >= 0.  Every cell of a segment has y >= int(start.y),
 // and a jump sets start.y += RN(dy * c) with c = the entry intersect plus the
 // jump.  A negative entry intersect either belongs to an axis whose step
 // leaves the grid (the walk ends before any jump), or comes from a zero
@@ -310,8 +310,12 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
     const float d1 = (M[1] * rx + M[5] * ry) + (M[9] * rz + M[13] * 0.0f);
     const float d2 = (M[2] * rx + M[6] * ry) + (M[10] * rz + M[14] * 0.0f);
 
+    // The primary ray may take the cutoff when its step count is not an
+    // output of this draw (no depth view, no step map): a ray that stops
+    // there misses, as the reference's walk would.
     Hit h;
-    cast_ray<IX>(g, f.cam_pos[0], f.cam_pos[1], f.cam_pos[2], d0, d1, d2, F.render_lim, h);
+    cast_ray<IX>(g, f.cam_pos[0], f.cam_pos[1], f.cam_pos[2], d0, d1, d2, F.render_lim, h,
+                 (F.primary_cutoff != 0 && d1 >= 0.0f) ? top : INT32_MAX);
     uint32_t iters = h.iters;
     uint32_t rays = 1;
     uint32_t eval = 0, occ = 0;

@@ -51,6 +51,9 @@ struct FrameDesc {
     uint32_t light_valid;
     float light_rinv[VRT_MAX_LOCAL_LIGHTS];  // RN(1 / radius)
     float inv_w, inv_h;                      // RN(1 / width), RN(1 / height) of the target
+    // Primary rays may stop above every solid voxel too: only when nothing
+    // shows their step count (no depth view, no step map in the target).
+    int32_t primary_cutoff;
 };
 
 // Ray/step counters: kCounterSlots slots of one 128-byte line each
@@ -172,7 +175,7 @@ int solid_top_full(vrt_ctx* c);  // recomputes *grid.top over the whole grid
 
 namespace vrt {
 int world_generate_device(const GridDesc& g, const vrt_world_params& p, hipStream_t s);
-FrameDesc make_frame_desc(const vrt_frame& f, int32_t width, int32_t height);
+FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target);
 int check_frame(const vrt_frame& f);
 int check_target(const vrt_target& t);
 }  // namespace vrt

@@ -288,9 +288,10 @@ class Event:
 
 # --------------------------------------------------------------- frame buffers
 class Target:
-    """Device frame buffers for one view: rgb (+ depth, + aux maps)."""
+    """Device frame buffers for one view: rgb (+ depth, + aux maps; steps=False
+    leaves out the primary step map, which lets primary rays use the cutoff)."""
 
-    def __init__(self, width, height, depth=True, aux=False):
+    def __init__(self, width, height, depth=True, aux=False, steps=True):
         self.width, self.height = int(width), int(height)
         n = self.width * self.height
         self.rgb = DeviceBuffer(3 * n)
@@ -301,7 +302,7 @@ class Target:
             self.face = DeviceBuffer(n)
             self.eval = DeviceBuffer(4 * n)
             self.occ = DeviceBuffer(4 * n)
-            self.steps = DeviceBuffer(4 * n)
+            self.steps = DeviceBuffer(4 * n) if steps else None
 
     def struct(self):
         t = vrt_target()
@@ -310,7 +311,8 @@ class Target:
         t.depth = self.depth.addr if self.depth else None
         if self.aux:
             t.hit, t.face = self.hit.addr, self.face.addr
-            t.shadow_eval, t.shadow_occ, t.steps = self.eval.addr, self.occ.addr, self.steps.addr
+            t.shadow_eval, t.shadow_occ = self.eval.addr, self.occ.addr
+            t.steps = self.steps.addr if self.steps else None
         return t
 
     def download(self):
@@ -323,7 +325,8 @@ class Target:
             out["face"] = self.face.download(np.uint8, (h, w))
             out["shadow_eval"] = self.eval.download(np.uint32, (h, w))
             out["shadow_occ"] = self.occ.download(np.uint32, (h, w))
-            out["steps"] = self.steps.download(np.uint32, (h, w))
+            if self.steps:
+                out["steps"] = self.steps.download(np.uint32, (h, w))
         return out
 
 
