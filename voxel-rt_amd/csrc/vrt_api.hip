@@ -183,18 +183,18 @@ int upload_table(vrt_ctx* c) {
 
 void release(vrt_ctx* c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->own_stream) hipStreamSynchronize(c->own_stream);
-    if (c->grid.vox) hipFree(c->grid.vox);
-    if (c->grid.top) hipFree(c->grid.top);
-    if (c->counters) hipFree(c->counters);
-    if (c->table_packed) hipFree(c->table_packed);
-    if (c->table_dist) hipFree(c->table_dist);
-    if (c->table_rows) hipFree(c->table_rows);
-    if (c->table_rows_dist) hipFree(c->table_rows_dist);
-    if (c->bricks) hipFree(c->bricks);
-    if (c->solid_bits) hipFree(c->solid_bits);
-    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    (void)hipSetDevice(c->device);
+    if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+    if (c->grid.vox) (void)hipFree(c->grid.vox);
+    if (c->grid.top) (void)hipFree(c->grid.top);
+    if (c->counters) (void)hipFree(c->counters);
+    if (c->table_packed) (void)hipFree(c->table_packed);
+    if (c->table_dist) (void)hipFree(c->table_dist);
+    if (c->table_rows) (void)hipFree(c->table_rows);
+    if (c->table_rows_dist) (void)hipFree(c->table_rows_dist);
+    if (c->bricks) (void)hipFree(c->bricks);
+    if (c->solid_bits) (void)hipFree(c->solid_bits);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
 
@@ -641,7 +641,7 @@ int vrt_selftest_reciprocal(vrt_ctx* ctx, uint64_t first, uint64_t count, uint64
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_fail(e, "selftest copy");
     }
-    hipFree(bad);
+    (void)hipFree(bad);
     if (rc) return rc;
     *mismatches = h;
     return VRT_OK;
@@ -660,7 +660,7 @@ int vrt_selftest_division(vrt_ctx* ctx, uint32_t d_begin, uint32_t d_count, uint
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e != hipSuccess) rc = hip_fail(e, "selftest copy");
     }
-    hipFree(bad);
+    (void)hipFree(bad);
     if (rc) return rc;
     *mismatches = h;
     return VRT_OK;

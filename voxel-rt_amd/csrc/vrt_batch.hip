@@ -44,14 +44,14 @@ int check_config(const vrt_batch_config& c) {
 
 void destroy_plan(vrt_batch* b) {
     if (!b) return;
-    if (b->ctx) hipSetDevice(b->ctx->device);
-    if (b->capture) hipStreamSynchronize(b->capture);
-    if (b->ctx) hipStreamSynchronize(b->ctx->stream);
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    if (b->views_dev) hipFree(b->views_dev);
-    if (b->queue) hipFree(b->queue);
-    if (b->capture) hipStreamDestroy(b->capture);
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    if (b->capture) (void)hipStreamSynchronize(b->capture);
+    if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->views_dev) (void)hipFree(b->views_dev);
+    if (b->queue) (void)hipFree(b->queue);
+    if (b->capture) (void)hipStreamDestroy(b->capture);
     delete b;
 }
 
@@ -192,12 +192,15 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, int32_t n, int32_t r
             std::vector<double> ms;
             rc = vrt_batch_launch(b);  // warm-up
             for (int32_t r = 0; r < reps && !rc; r++) {
-                hipEventRecord(e0, ctx->stream);
+                if (hipEventRecord(e0, ctx->stream) != hipSuccess) {
+                    rc = fail(VRT_ERR_HIP, "event record");
+                    break;
+                }
                 rc = vrt_batch_launch(b);
-                hipEventRecord(e1, ctx->stream);
-                if (hipEventSynchronize(e1) != hipSuccess) rc = fail(VRT_ERR_HIP, "event sync");
                 float t = 0.0f;
-                hipEventElapsedTime(&t, e0, e1);
+                if (!rc && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+                    rc = fail(VRT_ERR_HIP, "autotune timing");
                 ms.push_back(t);
             }
             vrt_batch_destroy(b);
@@ -207,8 +210,8 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, int32_t n, int32_t r
         }
         if (rc) break;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     if (rc) return rc;
     auto it = std::min_element(all.begin(), all.end(),
                                [](const vrt_tune_entry& a, const vrt_tune_entry& b) { return a.ms < b.ms; });

@@ -108,7 +108,7 @@ int alloc_target(vrt_server* s, Request& r) {
         // keep cached plus live buffers within the budget
         while (!s->pool.empty() && s->pool_bytes + s->bytes_in_use + r.bytes > s->cfg.framebuffer_budget) {
             auto big = std::prev(s->pool.end());
-            hipFree(big->second);
+            (void)hipFree(big->second);
             s->pool_bytes -= big->first;
             s->pool.erase(big);
         }
@@ -252,21 +252,21 @@ int admit(vrt_server* s, int* launched) {
 
 void destroy_server(vrt_server* s) {
     if (!s) return;
-    if (s->ctx) hipSetDevice(s->ctx->device);
+    if (s->ctx) (void)hipSetDevice(s->ctx->device);
     for (auto& sl : s->slots) {
-        if (sl.stream) hipStreamSynchronize(sl.stream);
-        if (sl.done) hipEventDestroy(sl.done);
-        if (sl.views_dev) hipFree(sl.views_dev);
-        if (sl.views_host) hipHostFree(sl.views_host);
-        if (sl.queue) hipFree(sl.queue);
-        if (sl.stream) hipStreamDestroy(sl.stream);
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.views_dev) (void)hipFree(sl.views_dev);
+        if (sl.views_host) (void)hipHostFree(sl.views_host);
+        if (sl.queue) (void)hipFree(sl.queue);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
-    if (s->edit_stream) hipStreamSynchronize(s->edit_stream);
+    if (s->edit_stream) (void)hipStreamSynchronize(s->edit_stream);
     for (auto& kv : s->reqs)
-        if (kv.second.mem) hipFree(kv.second.mem);
-    for (auto& kv : s->pool) hipFree(kv.second);
-    if (s->edit_done) hipEventDestroy(s->edit_done);
-    if (s->edit_stream) hipStreamDestroy(s->edit_stream);
+        if (kv.second.mem) (void)hipFree(kv.second.mem);
+    for (auto& kv : s->pool) (void)hipFree(kv.second);
+    if (s->edit_done) (void)hipEventDestroy(s->edit_done);
+    if (s->edit_stream) (void)hipStreamDestroy(s->edit_stream);
     delete s;
 }
 

@@ -277,7 +277,7 @@ int world_generate_device(const GridDesc& g, const vrt_world_params& p, hipStrea
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    hipFree(d);
+    (void)hipFree(d);
     if (e != hipSuccess) return hip_fail(e, "world generation");
     return VRT_OK;
 }
