@@ -4,6 +4,7 @@ placeLocalLight slot rule, world generator, error paths) behave; no GPU work."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -87,3 +88,38 @@ def test_world_params_rejected(vrt):
     p.height_max = 2  # below height_min
     out = np.empty(64 * 48 * 40, np.int32)
     assert vrt.lib().vrt_world_generate(C.byref(p), out.ctypes.data_as(C.c_void_p), 1) == vrt.VRT_ERR_INVALID
+
+
+C_CLIENT = r"""
+#include <stdio.h>
+#include "vrt.h"
+int main(void) {
+    vrt_frame f;
+    float m[16];
+    if (vrt_frame_defaults(&f, 512, 800, 600) != VRT_OK) return 1;
+    if (vrt_place_local_light(&f, 10.0f, 20.0f, 30.0f, 0.5f) != 0) return 2; /* slot 0 */
+    if (vrt_camera_matrix(0.25f, 1.5f, m) != VRT_OK) return 3;
+    printf("%zu %zu %.9g %.9g %.9g %.9g\n", sizeof(vrt_frame), sizeof(vrt_target), f.cam_pos[0],
+           f.lights[0].pos[1], f.lights[0].rgb[0], m[5]);
+    return 0;
+}
+"""
+
+
+def test_header_is_plain_c(vrt, tmp_path):
+    """include/vrt.h compiles as C99 with warnings as errors, a C program links
+    libvrt.so and calls its host-side helpers, and the struct sizes equal the
+    ctypes mirror's."""
+    src = tmp_path / "client.c"
+    src.write_text(C_CLIENT)
+    exe = tmp_path / "client"
+    libdir = os.path.dirname(vrt.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe), "-L", libdir, "-lvrt", "-Wl,-rpath," + libdir,
+                    "-Wl,--allow-shlib-undefined"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert int(out[0]) == C.sizeof(vrt.vrt_frame) and int(out[1]) == C.sizeof(vrt.vrt_target)
+    f = vrt.frame_defaults(512, 800, 600)
+    assert vrt.lib().vrt_place_local_light(C.byref(f), 10.0, 20.0, 30.0, 0.5) == 0
+    assert float(out[2]) == f.cam_pos[0] and float(out[3]) == f.lights[0].pos[1] and float(out[4]) == f.lights[0].rgb[0]
+    assert np.float32(float(out[5])) == np.float32(vrt.camera_matrix(0.25, 1.5)[5])
