@@ -9,7 +9,7 @@ Prints one JSON line.  Besides the headline value it reports the dominant
 kernel's roofline (algorithmic bytes per launch / its average duration from
 HIP events on the launch stream, against 8 TB/s; and its VALU lane-operations
 per launch, from the committed counter summary, against the nominal VALU
-rate), the same frames with the shadow cutoff off, the CPU oracle timed on
+rate), the same frames with the cutoff off, the CPU oracle timed on
 this host as a baseline, and, unless --no-extra, the batched form (64 views of
 mixed resolution in one persistent launch vs 64 plain launches), the served
 form (p50/p99 latency under a Poisson request stream with edits) and the
@@ -124,7 +124,7 @@ def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
         valu = {"achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "T lane-ops/s",
                 "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops,
                 "issue_frac_profiled": profiled("valu_issue_frac")}
-    # The same frames with the shadow cutoff off: every output is identical,
+    # The same frames with the cutoff off: every output is identical,
     # the DDA walks are the reference's full length (its iteration count).
     full = None
     if cutoff_off_leg:
@@ -142,8 +142,8 @@ def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
 
 
 def plain_cutoff_off(vrt, ctx, sc, ts, steps):
-    """The plain frame again with the shadow cutoff off (shadow rays walk the
-    reference's full length): the outputs are the same, the DDA-iteration
+    """The plain frame again with the cutoff off (shadow and primary rays walk
+    the reference's full length): the outputs are the same, the DDA-iteration
     total is the reference's."""
     ctx.set_shadow_cutoff(False)
     ctx.draw(ts, frame=sc.frame)
@@ -421,7 +421,7 @@ def main():
     sc, ctx, plain = plain_bench(vrt, args.steps, args.warmup, cutoff_off_leg=args.only != "plain")
     log("plain: %.1f Mrays/s, %.3f ms/frame" % (plain["value"], plain["kernel_ms"]))
     if plain["shadow_cutoff_off"]:
-        log("plain with the shadow cutoff off: %.1f Mrays/s, %.3f ms/frame" % (
+        log("plain with the cutoff off: %.1f Mrays/s, %.3f ms/frame" % (
             plain["shadow_cutoff_off"]["value"], plain["shadow_cutoff_off"]["kernel_ms"]))
     res = {
         "metric": METRIC, "value": plain["value"], "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
