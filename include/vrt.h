@@ -128,7 +128,8 @@ int vrt_grid_modified(vrt_ctx* ctx);
 /* Cutoff (on by default): a shadow ray that is above every solid voxel of the
  * grid and not heading down stops there as a miss instead of walking on to the
  * grid's edge or its budget; so does a primary ray when nothing shows its step
- * count (the frame is not the depth view and the target has no step map).
+ * count (the frame is not the depth view and the target has no step map), and
+ * a ray of vrt_cast_rays called without a steps array.
  * Every output equals the reference's either way; only vrt_stats.dda_steps
  * (iterations executed) differs: with the cutoff off it equals the
  * reference's count.  Applies to later draws and to batches created later.
@@ -213,7 +214,9 @@ int vrt_read_stats(vrt_ctx* ctx, vrt_stats* out, int32_t reset);
 /* castRay (src/fshader.glsl:59-129) for n independent rays: start[3*i],
  * dir[3*i] (device float), budget dist[i] (device int32) and step cap
  * render_dist.  Outputs (device, optional): hit index or -1, hit position,
- * face code of the last step, loop iterations. */
+ * face code of the last step, loop iterations.  Without a steps array the
+ * rays take the cutoff (vrt_set_shadow_cutoff); the other outputs are the
+ * same either way. */
 int vrt_cast_rays(vrt_ctx* ctx, int64_t n, const float* start, const float* dir,
                   const int32_t* dist, int32_t render_dist,
                   int64_t* hit, float* hit_pos, uint8_t* face, uint32_t* steps);

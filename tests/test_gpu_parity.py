@@ -446,6 +446,33 @@ def test_degenerate_worlds_and_ragged_frames(gpu, orc, idx):
     ctx.close()
 
 
+def test_cast_rays_cutoff_without_steps(gpu, orc):
+    """vrt_cast_rays without a steps array lets rays heading up stop above
+    every solid voxel: hit, position and face equal the run with steps (which
+    walks them to the end) and the oracle's."""
+    sc = gpu.make_scene("tiny", seed=3)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    X, Y, Z = sc.dims
+    rng = np.random.default_rng(5)
+    n = 20000
+    start = np.stack([rng.uniform(0, X, n), rng.uniform(Y * 0.3, Y, n), rng.uniform(0, Z, n)], 1).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d[:, 1] = np.abs(d[:, 1]) * np.where(rng.random(n) < 0.7, 1.0, -1.0)  # mostly upward
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    dist = np.full(n, 384, np.int32)
+    r = orc.cast_rays(wdf, sc.dims, start, d, dist, 384)
+    full = ctx.cast_rays(start, d, dist, 384)
+    cut = ctx.cast_rays(start, d, dist, 384, steps=False)
+    for k in ("hit", "face"):
+        assert np.array_equal(full[k], r[k]) and np.array_equal(cut[k], r[k]), k
+    assert same_floats(cut["hit_pos"], r["hit_pos"]) and np.array_equal(full["steps"], r["steps"])
+    assert int((r["hit"] == -1).sum()) > n // 4
+    ctx.close()
+
+
 def test_pin_rays_match_oracle(gpu, orc):
     """The closed-form rays that pin the oracle (start-cell truncation, exactly
     zero direction components, the 1e-6 step bias) through the kernels."""

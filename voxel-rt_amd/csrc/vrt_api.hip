@@ -625,7 +625,8 @@ int vrt_cast_rays(vrt_ctx* ctx, int64_t n, const float* start, const float* dir,
     if (int rc = check_ctx(ctx)) return rc;
     if (n < 0 || (n > 0 && (!start || !dir || !dist))) return fail(VRT_ERR_INVALID, "invalid ray arrays");
     if (render_dist < 0 || render_dist > (1 << 24)) return fail(VRT_ERR_INVALID, "render_dist must be in [0, 2^24]");
-    return launch_cast_rays(ctx->grid, n, start, dir, dist, render_dist, hit, hit_pos, face, steps, ctx->stream);
+    return launch_cast_rays(render_grid(ctx), n, start, dir, dist, render_dist, hit, hit_pos, face, steps,
+                            ctx->stream);
 }
 
 int vrt_selftest_reciprocal(vrt_ctx* ctx, uint64_t first, uint64_t count, uint64_t* mismatches) {

@@ -88,13 +88,16 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
                                                    const float* __restrict__ dir,
                                                    const int32_t* __restrict__ dist, float render_lim,
                                                    int64_t* hit, float* hit_pos, uint8_t* face, uint32_t* steps) {
+    const int32_t top = g.top ? *g.top : INT32_MAX;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float lim = (float)dist[i];
     lim = lim < render_lim ? lim : render_lim;
+    // the cutoff applies when the caller does not ask for the step counts
+    const float dy = dir[3 * i + 1];
     dev::Hit h;
-    dev::cast_ray<IX>(g, start[3 * i], start[3 * i + 1], start[3 * i + 2], dir[3 * i], dir[3 * i + 1], dir[3 * i + 2],
-                  lim, h);
+    dev::cast_ray<IX>(g, start[3 * i], start[3 * i + 1], start[3 * i + 2], dir[3 * i], dy, dir[3 * i + 2], lim, h,
+                      (steps == nullptr && dy >= 0.0f) ? top : INT32_MAX);
     const bool hitp = h.index != -1;
     if (hit) hit[i] = h.index;
     if (hit_pos) {

@@ -448,7 +448,9 @@ class Context:
         call("vrt_selftest_reciprocal", self.h, int(first), int(count), C.byref(bad))
         return bad.value
 
-    def cast_rays(self, start, direction, dist, render_dist):
+    def cast_rays(self, start, direction, dist, render_dist, steps=True):
+        """castRay for each row; steps=False leaves out the iteration counts
+        (the rays may then take the cutoff; the other outputs are the same)."""
         start = np.ascontiguousarray(start, dtype=np.float32).reshape(-1, 3)
         direction = np.ascontiguousarray(direction, dtype=np.float32).reshape(-1, 3)
         dist = np.ascontiguousarray(dist, dtype=np.int32).reshape(-1)
@@ -457,12 +459,16 @@ class Context:
         bs.upload(start)
         bd.upload(direction)
         bl.upload(dist)
-        bh, bp, bf, bt = DeviceBuffer(8 * n), DeviceBuffer(12 * n), DeviceBuffer(n), DeviceBuffer(4 * n)
+        bh, bp, bf = DeviceBuffer(8 * n), DeviceBuffer(12 * n), DeviceBuffer(n)
+        bt = DeviceBuffer(4 * n) if steps else None
         call("vrt_cast_rays", self.h, n, bs.addr, bd.addr, bl.addr, int(render_dist), bh.addr, bp.addr,
-             bf.addr, bt.addr)
+             bf.addr, bt.addr if bt else None)
         self.sync()
-        return {"hit": bh.download(np.int64, (n,)), "hit_pos": bp.download(np.float32, (n, 3)),
-                "face": bf.download(np.uint8, (n,)), "steps": bt.download(np.uint32, (n,))}
+        out = {"hit": bh.download(np.int64, (n,)), "hit_pos": bp.download(np.float32, (n, 3)),
+               "face": bf.download(np.uint8, (n,))}
+        if bt:
+            out["steps"] = bt.download(np.uint32, (n,))
+        return out
 
 
 # ------------------------------------------------------------- frame helpers
