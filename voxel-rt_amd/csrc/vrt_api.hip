@@ -86,6 +86,21 @@ float sqrt_le_limit(float r) {
     return t;
 }
 
+// Largest float a >= 0 with a * a <= t (one IEEE product, as on the device);
+// -inf for NaN or negative t, +inf for t = +inf.
+float square_le_limit(float t) {
+    if (!(t >= 0.0f)) return -INFINITY;
+    if (t == INFINITY) return INFINITY;
+    float a = std::sqrt(t);
+    while (a > 0.0f && !(a * a <= t)) a = std::nextafter(a, -INFINITY);
+    for (;;) {
+        const float up = std::nextafter(a, INFINITY);
+        if (!(up * up <= t)) break;
+        a = up;
+    }
+    return a;
+}
+
 FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target) {
     const int32_t width = target.width, height = target.height;
     FrameDesc d;
@@ -98,6 +113,7 @@ FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target) {
     for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
         const vrt_local_light& L = f.lights[i];
         d.light_dd_max[i] = sqrt_le_limit(L.radius);
+        d.light_axis_max[i] = square_le_limit(d.light_dd_max[i]);
         d.light_rinv[i] = 1.0f / L.radius;
         if (L.pos[0] >= 0.0f && L.pos[1] >= 0.0f && L.pos[2] >= 0.0f) d.light_valid |= 1u << i;
     }

@@ -132,8 +132,11 @@ struct Hit {
 struct IdxB {
     using T = uint32_t;  // byte offset
     __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
-        return (uint32_t)(x * 4) + (uint32_t)__mul24(y, (int32_t)(g.X * 4)) +
-               (uint32_t)__mul24(z, (int32_t)(g.XY * 4));
+        // a 24-bit multiply, a 24-bit multiply-add and a shift-add (the
+        // compiler would make the sum a three-input add after a separate shift)
+        uint32_t t = (uint32_t)__mul24(y, (int32_t)(g.X * 4));
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(t) : "v"(z), "s"((int32_t)(g.XY * 4)), "v"(t));
+        return ((uint32_t)x << 2) + t;
     }
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
         return 4u * (axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.X : (uint32_t)sz * (uint32_t)g.XY));
@@ -385,9 +388,12 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
             if (over) break;
             const int i = __builtin_ctz(rem);
             const vrt_local_light& L = f.lights[i];
-            const float vx0 = L.pos[0] - hx, vy0 = L.pos[1] - hy, vz0 = L.pos[2] - hz;
+            // length <= radius (:171), decided without the square root; the
+            // x distance alone rules most lights out for the whole wave
+            const float vx0 = L.pos[0] - hx;
+            if (!(fabsf(vx0) <= F.light_axis_max[i])) continue;
+            const float vy0 = L.pos[1] - hy, vz0 = L.pos[2] - hz;
             const float dd = dot3(vx0, vy0, vz0, vx0, vy0, vz0);
-            // length <= radius (:171), decided without the square root
             if (!(dd <= F.light_dd_max[i])) continue;
             const float dist = sqrtf(dd);
             const float inv = rcp_rn(dist);

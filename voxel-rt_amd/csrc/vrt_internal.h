@@ -46,6 +46,10 @@ struct FrameDesc {
     // Per light: the largest float t with sqrt_rn(t) <= radius, so that
     // length(v) <= radius  <=>  dot(v, v) <= t  (sqrt_rn is monotone).
     float light_dd_max[VRT_MAX_LOCAL_LIGHTS];
+    // Per light: the largest float a >= 0 with RN(a * a) <= light_dd_max.
+    // dot(v, v) >= RN(v.x * v.x) (a sum of non-negative terms, each rounding
+    // monotone), so |v.x| > a already puts the light out of range.
+    float light_axis_max[VRT_MAX_LOCAL_LIGHTS];
     // Bit i set when light i is placed: every coordinate >= 0 (unused slots
     // hold -1, src/render.cpp:304-311; NaN compares false).
     uint32_t light_valid;
