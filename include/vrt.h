@@ -117,13 +117,14 @@ int vrt_destroy(vrt_ctx* ctx);
 int vrt_set_stream(vrt_ctx* ctx, void* stream);
 void* vrt_get_stream(vrt_ctx* ctx);
 int vrt_sync(vrt_ctx* ctx);
-/* Device pointer of the grid (int32_t[X*Y*Z]).  After writing solid voxels
- * through it, call vrt_grid_modified before the next draw. */
+/* Device pointer of the grid (int32_t[X*Y*Z]).  After writing through it,
+ * call vrt_grid_modified before the next draw. */
 int vrt_grid_device_ptr(vrt_ctx* ctx, int32_t** out);
-/* Re-derives what the context keeps about the grid's contents (the highest y
- * of any solid voxel, used by the shadow cutoff) after writes made through
- * vrt_grid_device_ptr.  The library's own uploads and edits keep it current.
- * No counterpart in the reference, whose grid has no device-side writers. */
+/* Re-derives what the context keeps about the grid's contents after writes
+ * made through vrt_grid_device_ptr: the highest y of any solid voxel (used by
+ * the shadow cutoff) and the render kernels' padded copy of the grid.  The
+ * library's own uploads and edits keep both current.  No counterpart in the
+ * reference, whose grid has no device-side writers. */
 int vrt_grid_modified(vrt_ctx* ctx);
 /* Cutoff (on by default): a shadow ray that is above every solid voxel of the
  * grid and not heading down stops there as a miss instead of walking on to the

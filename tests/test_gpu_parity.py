@@ -353,6 +353,51 @@ def test_cast_rays_entering_from_one_cell_outside(gpu, orc, dims):
     ctx.close()
 
 
+def test_cast_rays_near_and_past_the_grid_faces(gpu, orc):
+    """Walks that start from 3 cells inside to 5 cells outside a face (either
+    side, every axis), pointing in, out, or nearly along it; walks that jump
+    out of the grid near a face; and unnormalized directions, whose
+    depth-field jumps land up to hundreds of cells away.  Every exit must come
+    at the reference's iteration (a cell off the grid ends the walk, one back
+    inside continues it)."""
+    rng = np.random.default_rng(31)
+    dims = (45, 38, 41)
+    X, Y, Z = dims
+    world = np.where(rng.random(X * Y * Z) < 0.03, rng.integers(0, 1 << 24, X * Y * Z), -1).astype(np.int32)
+    ctx = gpu.Context(dims)
+    ctx.upload(world)
+    ctx.build_depth_field()
+    wdf = ctx.download()
+    starts, dirs = [], []
+    n = 400
+    for axis in range(3):
+        for low in (True, False):
+            for off in (-3.5, -0.5, 0.5, 1.5, 2.5, 5.5):
+                st = np.stack([rng.uniform(-0.5, dims[k] + 0.5, n) for k in range(3)], 1)
+                st[:, axis] = -off if low else dims[axis] - 1 + off
+                d = rng.normal(size=(n, 3))
+                # a third pointing in, a third out, a third along the face
+                sgn = (1.0 if low else -1.0) * np.where(np.arange(n) % 3 == 0, 1.0,
+                                                          np.where(np.arange(n) % 3 == 1, -1.0, 0.05))
+                d[:, axis] = np.abs(d[:, axis]) * sgn
+                d /= np.linalg.norm(d, axis=1, keepdims=True)
+                d *= np.where(np.arange(n) % 5 == 0, 40.0, 1.0)[:, None]  # some unnormalized
+                starts.append(st)
+                dirs.append(d)
+    start = np.concatenate(starts).astype(np.float32)
+    d = np.concatenate(dirs).astype(np.float32)
+    dist = rng.integers(1, 200, start.shape[0]).astype(np.int32)
+    for budget in (384, 1 << 24):
+        g = ctx.cast_rays(start, d, dist if budget == 384 else np.full_like(dist, budget), budget)
+        r = orc.cast_rays(wdf, dims, start, d, dist if budget == 384 else np.full_like(dist, budget), budget)
+        for k in ("hit", "face", "steps"):
+            bad = np.nonzero(g[k] != r[k])[0]
+            assert bad.size == 0, "%s differs for %d rays, first %d" % (k, bad.size, bad[0])
+        assert same_floats(g["hit_pos"], r["hit_pos"])
+        assert int((g["hit"] >= 0).sum()) > 1000
+    ctx.close()
+
+
 def test_cast_rays_extreme_values_match_oracle(gpu, orc):
     """int(float) at NaN, +-inf and beyond int32 range, zero and denormal
     direction components, huge budgets: the conversions saturate and NaN goes

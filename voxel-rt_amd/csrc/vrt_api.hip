@@ -202,6 +202,7 @@ void release(vrt_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
     if (c->grid.vox) (void)hipFree(c->grid.vox);
+    if (c->rgrid.vox) (void)hipFree(c->rgrid.vox);
     if (c->grid.top) (void)hipFree(c->grid.top);
     if (c->counters) (void)hipFree(c->counters);
     if (c->table_packed) (void)hipFree(c->table_packed);
@@ -258,6 +259,19 @@ int solid_top_full(vrt_ctx* c) {
     return launch_solid_top(c->grid, 0, 0, 0, c->grid.X, c->grid.Y, c->grid.Z, true, c->stream);
 }
 
+int refresh_render(vrt_ctx* c, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1, int64_t z1) {
+    x0 = std::max<int64_t>(x0, 0);
+    y0 = std::max<int64_t>(y0, 0);
+    z0 = std::max<int64_t>(z0, 0);
+    x1 = std::min<int64_t>(x1, c->grid.X);
+    y1 = std::min<int64_t>(y1, c->grid.Y);
+    z1 = std::min<int64_t>(z1, c->grid.Z);
+    if (x1 <= x0 || y1 <= y0 || z1 <= z0) return VRT_OK;
+    return launch_copy_box(c->grid, c->rgrid, x0, y0, z0, x1 - x0, y1 - y0, z1 - z0, c->stream);
+}
+
+int refresh_render_all(vrt_ctx* c) { return refresh_render(c, 0, 0, 0, c->grid.X, c->grid.Y, c->grid.Z); }
+
 }  // namespace vrt
 
 using namespace vrt;
@@ -311,6 +325,24 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
     c->grid.Z = (uint32_t)Z;
     c->grid.XY = (uint64_t)X * (uint64_t)Y;
     c->grid.count = count;
+    c->grid.SX = (uint32_t)X;
+    c->grid.SXY = c->grid.XY;
+    c->grid.origin = 0;
+    {
+        const uint64_t PX = (uint64_t)X + 2 * kGuard, PY = (uint64_t)Y + 2 * kGuard, PZ = (uint64_t)Z + 2 * kGuard;
+        GridDesc& r = c->rgrid;
+        r = c->grid;
+        r.SX = (uint32_t)PX;
+        r.SXY = PX * PY;
+        r.count = PX * PY * PZ;
+        r.origin = (uint64_t)kGuard * (1 + PX + PX * PY);
+        r.vox = nullptr;
+        if (hipMalloc(&r.vox, r.count * sizeof(int32_t)) != hipSuccess) {
+            (void)hipGetLastError();
+            r.vox = nullptr;
+            return bail(fail(VRT_ERR_NOMEM, "cannot allocate the render copy of the voxel grid"));
+        }
+    }
     if ((e = hipMalloc(&c->grid.top, sizeof(int32_t))) != hipSuccess) return bail(hip_fail(e, "hipMalloc top"));
     const size_t counter_bytes = (size_t)kCounterSlots * kCounterStride * sizeof(unsigned long long);
     if ((e = hipMalloc(&c->counters, counter_bytes)) != hipSuccess)
@@ -322,6 +354,8 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
     if ((e = hipMemsetAsync(c->counters, 0, counter_bytes, c->stream)) != hipSuccess)
         return bail(hip_fail(e, "hipMemsetAsync"));
     rc = launch_fill(c->grid.vox, count, -1, c->stream);
+    if (!rc) rc = launch_fill(c->rgrid.vox, c->rgrid.count, kGuardValue, c->stream);
+    if (!rc) rc = refresh_render_all(c);
     if (rc) return bail(rc);
     rc = launch_solid_top(c->grid, 0, 0, 0, 0, 0, 0, true, c->stream);  // no solid voxel yet: -1
     if (rc) return bail(rc);
@@ -373,6 +407,7 @@ int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels) {
     VRT_HIP(hipMemcpyAsync(ctx->grid.vox, host_voxels, ctx->grid.count * sizeof(int32_t), hipMemcpyHostToDevice,
                            ctx->stream));
     if (int rc = solid_top_full(ctx)) return rc;
+    if (int rc = refresh_render_all(ctx)) return rc;
     VRT_HIP(hipStreamSynchronize(ctx->stream));
     return VRT_OK;
 }
@@ -381,12 +416,14 @@ int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels) {
     if (int rc = check_ctx(ctx)) return rc;
     if (!device_voxels) return fail(VRT_ERR_INVALID, "null voxels");
     if (int rc = launch_copy(ctx->grid.vox, device_voxels, ctx->grid.count, ctx->stream)) return rc;
-    return solid_top_full(ctx);
+    if (int rc = solid_top_full(ctx)) return rc;
+    return refresh_render_all(ctx);
 }
 
 int vrt_grid_modified(vrt_ctx* ctx) {
     if (int rc = check_ctx(ctx)) return rc;
-    return solid_top_full(ctx);
+    if (int rc = solid_top_full(ctx)) return rc;
+    return refresh_render_all(ctx);
 }
 
 int vrt_set_shadow_cutoff(vrt_ctx* ctx, int32_t enable) {
@@ -430,6 +467,7 @@ int vrt_upload_region(vrt_ctx* ctx, const int32_t* host_voxels, int32_t x0, int3
     VRT_HIP(hipMemcpy3DAsync(&p, ctx->stream));
     // the region can only have gained solid voxels above the old bound
     if (int rc = launch_solid_top(g, xs, ys, zs, xe + 1, ye, ze, false, ctx->stream)) return rc;
+    if (int rc = refresh_render(ctx, xs, ys, zs, xe + 1, ye, ze)) return rc;
     VRT_HIP(hipStreamSynchronize(ctx->stream));
     return VRT_OK;
 }
@@ -469,13 +507,16 @@ int vrt_build_depth_field(vrt_ctx* ctx) {
             return fail(VRT_ERR_NOMEM, "cannot allocate the depth-field build's solidity bitmap");
         }
     }
-    return launch_build_depth_field(g, ctx->table, ctx->bricks, ctx->solid_bits, ctx->stream);
+    if (int rc = launch_build_depth_field(g, ctx->table, ctx->bricks, ctx->solid_bits, ctx->stream)) return rc;
+    return refresh_render_all(ctx);
 }
 
 // fixDepthField, src/render.cpp:226-253, over a box.
 int vrt_fix_depth_field_box(vrt_ctx* ctx, int32_t x0, int32_t y0, int32_t z0, int32_t x1, int32_t y1, int32_t z1) {
     if (int rc = check_ctx(ctx)) return rc;
-    return launch_fix_depth_box(ctx->grid, ctx->table, x0, y0, z0, x1, y1, z1, 0, 0, 0, -1, ctx->stream);
+    if (int rc = launch_fix_depth_box(ctx->grid, ctx->table, x0, y0, z0, x1, y1, z1, 0, 0, 0, -1, ctx->stream))
+        return rc;
+    return refresh_render(ctx, x0, y0, z0, x1, y1, z1);
 }
 
 // removeSphere, src/level.cpp:30-56.
@@ -484,12 +525,18 @@ int vrt_remove_sphere(vrt_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t 
     if (radius < -3 || radius > 4096) return fail(VRT_ERR_INVALID, "radius must be in [-3, 4096]");
     if (int rc = launch_carve_sphere(ctx->grid, cx, cy, cz, radius, ctx->stream)) return rc;
     const int32_t R = radius + (VRT_DEPTH_FIELD_RADIUS >> 1);
-    if (R <= 0) return VRT_OK;
-    const int64_t x0 = (int64_t)cx - R, x1 = (int64_t)cx + R, y0 = (int64_t)cy - R, y1 = (int64_t)cy + R,
-                  z0 = (int64_t)cz - R, z1 = (int64_t)cz + R;
-    auto c32 = [](int64_t v) { return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v)); };
-    return launch_fix_depth_box(ctx->grid, ctx->table, c32(x0), c32(y0), c32(z0), c32(x1), c32(y1), c32(z1), cx, cy,
-                                cz, R, ctx->stream);
+    if (R > 0) {
+        const int64_t x0 = (int64_t)cx - R, x1 = (int64_t)cx + R, y0 = (int64_t)cy - R, y1 = (int64_t)cy + R,
+                      z0 = (int64_t)cz - R, z1 = (int64_t)cz + R;
+        auto c32 = [](int64_t v) { return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v)); };
+        if (int rc = launch_fix_depth_box(ctx->grid, ctx->table, c32(x0), c32(y0), c32(z0), c32(x1), c32(y1),
+                                          c32(z1), cx, cy, cz, R, ctx->stream))
+            return rc;
+    }
+    // the carve covers [c - |r|, c + |r|) at most, the repair [c - R, c + R)
+    const int64_t M = std::max<int64_t>(radius < 0 ? -(int64_t)radius : (int64_t)radius, R);
+    return refresh_render(ctx, (int64_t)cx - M, (int64_t)cy - M, (int64_t)cz - M, (int64_t)cx + M, (int64_t)cy + M,
+                          (int64_t)cz + M);
 }
 
 // placeVoxel, src/render.cpp:256-262.
@@ -497,7 +544,9 @@ int vrt_place_voxel(vrt_ctx* ctx, int32_t x, int32_t y, int32_t z, int32_t voxel
     if (int rc = check_ctx(ctx)) return rc;
     const GridDesc& g = ctx->grid;
     if (x < 0 || y < 0 || z < 0 || (uint32_t)x >= g.X || (uint32_t)y >= g.Y || (uint32_t)z >= g.Z) return VRT_OK;
-    return launch_set_voxel(g, (int64_t)x + (int64_t)g.X * y + (int64_t)g.XY * z, y, voxel, ctx->stream);
+    if (int rc = launch_set_voxel(g, (int64_t)x + (int64_t)g.X * y + (int64_t)g.XY * z, y, voxel, ctx->stream))
+        return rc;
+    return refresh_render(ctx, x, y, z, (int64_t)x + 1, (int64_t)y + 1, (int64_t)z + 1);
 }
 
 // destroyVoxel, src/render.cpp:265-271.
@@ -687,7 +736,8 @@ int vrt_world_generate_device(vrt_ctx* ctx, const vrt_world_params* p) {
     if (int rc = check_ctx(ctx)) return rc;
     if (!p) return fail(VRT_ERR_INVALID, "null params");
     if (int rc = world_generate_device(ctx->grid, *p, ctx->stream)) return rc;
-    return solid_top_full(ctx);
+    if (int rc = solid_top_full(ctx)) return rc;
+    return refresh_render_all(ctx);
 }
 
 }  // extern "C"

@@ -113,69 +113,104 @@ __device__ __forceinline__ uint8_t unorm8(float c) {
 
 // Result of one castRay.
 struct Hit {
-    int64_t index;   // voxel index or -1
+    int64_t index;   // word index of the hit voxel in the padded render copy, or -1
     float pos[3];    // hitPos (valid on a hit)
     int32_t axis;    // axis of the step into the hit voxel (-1 on a miss)
     int32_t step;    // step on that axis
     uint32_t iters;  // loop iterations (stepCount increments)
 };
 
-// Voxel index arithmetic, three forms: IdxB keeps the byte offset of the
-// voxel (4 x index) in 32 bits when the grid is at most 2^32 bytes and 4X,
-// 4X*Y < 2^23, so the gather adds it to the grid's base address in the load
-// itself; Idx32 keeps the index in 32 bits with signed 24-bit multiplies when
-// X*Y*Z <= 2^32 and X, X*Y < 2^23; Idx64 otherwise.  A start or jump cell may
-// lie outside the grid, and one of -1 on an axis can step into it, so make()
-// must be exact modulo 2^32 for negative coordinates too: the signed
-// multiplies are, for |y|, |z| < 2^23 (cells farther out cannot reach the
-// grid in one step, and their index is never read).
+// Clamps a cell coordinate to [-2, n + 1] (one v_med3_i32).  A walk indexes
+// the padded copy only through this: a cell two or more outside the grid
+// cannot enter it on the next step, and from -2 or n + 1 that step lands on a
+// guard cell (|kGuard| = 3) whatever its direction, as it lands outside the
+// grid from the true cell.
+__device__ __forceinline__ int32_t clamp_cell(int32_t v, uint32_t n) {
+    int32_t r;
+    asm("v_med3_i32 %0, %1, -2, %2" : "=v"(r) : "v"(v), "s"((int32_t)n + 1));
+    return r;
+}
+
+// Index arithmetic on the padded render copy, three forms: IdxB keeps the
+// byte offset of the word (4 x index) in 32 bits when the copy is at most
+// 2^32 bytes and 4 SX, 4 SXY < 2^23, so the gather adds it to the copy's base
+// address in the load itself; Idx32 keeps the word index in 32 bits with
+// signed 24-bit multiplies when the copy has at most 2^32 words and SX, SXY <
+// 2^23; Idx64 otherwise.  make() takes clamped cells (clamp_cell), so every
+// offset is non-negative and inside the copy, and y, z fit 24 bits.
+// ref_index() turns the word index of an in-grid cell back into the
+// reference's x + X*y + X*Y*z (src/fshader.glsl:36-39).
+template <typename U>
+__device__ __forceinline__ int64_t ref_index_of(const GridDesc& g, U L) {
+    const U z = L / (U)g.SXY;
+    const U r = L - z * (U)g.SXY;
+    const U y = r / (U)g.SX;
+    const U x = r - y * (U)g.SX;
+    return (int64_t)x + (int64_t)g.X * (int64_t)y + (int64_t)g.XY * (int64_t)z;
+}
 struct IdxB {
     using T = uint32_t;  // byte offset
     __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
-        // a 24-bit multiply, a 24-bit multiply-add and a shift-add (the
-        // compiler would make the sum a three-input add after a separate shift)
-        uint32_t t = (uint32_t)__mul24(y, (int32_t)(g.X * 4));
-        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(t) : "v"(z), "s"((int32_t)(g.XY * 4)), "v"(t));
-        return ((uint32_t)x << 2) + t;
+        // a shift-add and two 24-bit multiply-adds
+        uint32_t t = ((uint32_t)x << 2) + (uint32_t)g.origin * 4u;
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(t) : "v"(y), "s"((int32_t)(g.SX * 4)), "v"(t));
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(t) : "v"(z), "s"((int32_t)(g.SXY * 4)), "v"(t));
+        return t;
     }
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
-        return 4u * (axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.X : (uint32_t)sz * (uint32_t)g.XY));
+        return 4u * (axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.SX : (uint32_t)sz * (uint32_t)g.SXY));
     }
     __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) {
         return *(const int32_t*)((const char*)vox + i);
     }
-    __device__ __forceinline__ static int64_t index(T i) { return (int64_t)(i >> 2); }
+    __device__ __forceinline__ static int64_t word(T i) { return (int64_t)(i >> 2); }
+    __device__ __forceinline__ static int64_t ref_index(const GridDesc& g, int64_t w) {
+        return ref_index_of<uint32_t>(g, (uint32_t)w - (uint32_t)g.origin);
+    }
 };
 struct Idx32 {
     using T = uint32_t;
     __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
-        return (uint32_t)x + (uint32_t)__mul24(y, (int32_t)g.X) + (uint32_t)__mul24(z, (int32_t)g.XY);
+        uint32_t t = (uint32_t)x + (uint32_t)g.origin;
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(t) : "v"(y), "s"((int32_t)g.SX), "v"(t));
+        asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(t) : "v"(z), "s"((int32_t)g.SXY), "v"(t));
+        return t;
     }
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
-        return axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.X : (uint32_t)sz * (uint32_t)g.XY);
+        return axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.SX : (uint32_t)sz * (uint32_t)g.SXY);
     }
     __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) { return vox[i]; }
-    __device__ __forceinline__ static int64_t index(T i) { return (int64_t)i; }
+    __device__ __forceinline__ static int64_t word(T i) { return (int64_t)i; }
+    __device__ __forceinline__ static int64_t ref_index(const GridDesc& g, int64_t w) {
+        return ref_index_of<uint32_t>(g, (uint32_t)w - (uint32_t)g.origin);
+    }
 };
 struct Idx64 {
     using T = uint64_t;
     __device__ __forceinline__ static T make(const GridDesc& g, int32_t x, int32_t y, int32_t z) {
-        return (uint64_t)(int64_t)x + (uint64_t)g.X * (uint64_t)(int64_t)y + g.XY * (uint64_t)(int64_t)z;
+        return g.origin + (uint64_t)(int64_t)x + (uint64_t)g.SX * (uint64_t)(int64_t)y +
+               g.SXY * (uint64_t)(int64_t)z;
     }
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
         return axis == 0 ? (uint64_t)(int64_t)sx
-                         : (axis == 1 ? (uint64_t)(int64_t)sy * g.X : (uint64_t)(int64_t)sz * g.XY);
+                         : (axis == 1 ? (uint64_t)(int64_t)sy * g.SX : (uint64_t)(int64_t)sz * g.SXY);
     }
     __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) { return vox[i]; }
-    __device__ __forceinline__ static int64_t index(T i) { return (int64_t)i; }
+    __device__ __forceinline__ static int64_t word(T i) { return (int64_t)i; }
+    __device__ __forceinline__ static int64_t ref_index(const GridDesc& g, int64_t w) {
+        return ref_index_of<uint64_t>(g, (uint64_t)w - g.origin);
+    }
 };
 
 // castRay, src/fshader.glsl:59-129.  `lim` = min(float(dist), float(RENDER_DIST)):
 // the loop test `t < dist && t < RENDER_DIST` of :83 is `t < lim` for any t.
-// The voxel index of the current cell is kept incrementally (wrapping) and
-// is only used while the cell is inside the grid (per-axis bounds, :42-45):
-// between two in-grid cells every step was checked, so it never wrapped.
-// The axis choice is computed with selects so the wave does not branch.
+// The walk reads the padded render copy and keeps only the word offset of
+// the current cell, advanced by one step's stride per iteration; the cell
+// coordinates are formed at the start and after each jump (clamp_cell), never
+// per step.  A step off the grid (per-axis bounds, :42-45) lands on a guard
+// cell, whose NaN jump ends the walk in that iteration with the result and
+// iteration count of the reference's exit at :123-125.  The axis choice is
+// computed with selects so the wave does not branch.
 //
 // ytop (shadow rays, and primary rays whose step count nothing shows): an
 // upper bound on the y of every solid voxel, or INT32_MAX.  Let dy >= 0.  Every cell of a segment has y >= int(start.y),
@@ -195,7 +230,7 @@ template <typename IX>
 __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, float sz, float dx, float dy,
                                          float dz, float lim, Hit& h, int32_t ytop = INT32_MAX) {
     using T = typename IX::T;
-    int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
+    const int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
     const int32_t stx = isign(dx), sty = isign(dy), stz = isign(dz);
     const int32_t fx = stx > 0, fy = sty > 0, fz = stz > 0;
     const float ddx = rcp_rn(fabsf(dx + 0.000001f));
@@ -209,7 +244,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     const T sdx = IX::delta(g, stx, sty, stz, 0);
     const T sdy = IX::delta(g, stx, sty, stz, 1);
     const T sdz = IX::delta(g, stx, sty, stz, 2);
-    T idx = IX::make(g, cx, cy, cz);
+    T idx = IX::make(g, clamp_cell(cx, g.X), clamp_cell(cy, g.Y), clamp_cell(cz, g.Z));
     const int32_t* __restrict__ vox = g.vox;
 
     float trav = 0.0f;
@@ -222,12 +257,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
         const bool bx = (ix < iy) & (ix < iz);
         const bool by = (iy < ix) & (iy < iz);
         const bool bz = !(bx | by);
-        cx += bx ? stx : 0;
-        cy += by ? sty : 0;
-        cz += bz ? stz : 0;
         idx += bx ? sdx : (by ? sdy : sdz);
-        const bool inb = ((uint32_t)cx < g.X) & ((uint32_t)cy < g.Y) & ((uint32_t)cz < g.Z);
-        if (!inb) break;
         const int32_t v = IX::load(vox, idx);
         // currentIntersect: the intersect of the axis just stepped, before its update
         const float cur = bx ? ix : (by ? iy : iz);
@@ -236,24 +266,26 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             h.pos[1] = dy * cur + sy;
             h.pos[2] = dz * cur + sz;
             axis = bx ? 0 : (by ? 1 : 2);
-            result = IX::index(idx);
+            result = IX::word(idx);
             break;
         }
         if (v != -1) {
             // the jump recomputes every intersect, so the step's update is skipped
             const float toJump = -__int_as_float(v);
             trav += toJump;
+            // A jump that leaves `travelled` at or past the budget (or NaN:
+            // a guard cell) ends the walk at the loop test (:83); nothing the
+            // jump recomputes is read after that.
+            if (!(trav < lim)) break;
             const float c2 = cur + toJump;
             sx = dx * c2 + sx;
             sy = dy * c2 + sy;
             sz = dz * c2 + sz;
-            cx = f2i(sx);
-            cy = f2i(sy);
-            cz = f2i(sz);
+            const int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
             if (cy > ytop) break;
             div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok,
                     ix, iy, iz);
-            idx = IX::make(g, cx, cy, cz);
+            idx = IX::make(g, clamp_cell(cx, g.X), clamp_cell(cy, g.Y), clamp_cell(cz, g.Z));
         } else {
             // The axis just taken has step 0 (its direction component is +-0,
             // so its intersect is +-inf or NaN and adding its delta keeps it
@@ -329,7 +361,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
     const float ny = h.axis == 1 ? (float)(-h.step) : 0.0f;
     const float nz = h.axis == 2 ? (float)(-h.step) : 0.0f;
 
-    o.hit = h.index;
+    o.hit = hit ? IX::ref_index(g, h.index) : -1;
     o.face = hit ? (uint8_t)(h.axis * 3 + (h.step + 1)) : (uint8_t)VRT_FACE_MISS;
     o.steps = h.iters;
     if (hit) {

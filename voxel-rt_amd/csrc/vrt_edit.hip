@@ -14,6 +14,8 @@
 // otherwise the scan reads the neighbourhood's solidity from LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "vrt_internal.h"
 
 namespace vrt {
@@ -250,6 +252,20 @@ __global__ void __launch_bounds__(256) k_copy(int4* __restrict__ dst, const int4
     if (t < tail) dst_tail[t] = src_tail[t];
 }
 
+// One x-row segment of the box per (blockIdx.y, blockIdx.z) step; the y and z
+// loops cover boxes taller or deeper than the launch.
+__global__ void __launch_bounds__(256) k_copy_box(GridDesc src, GridDesc dst, int64_t x0, int64_t y0, int64_t z0,
+                                                  uint32_t nx, uint32_t ny, uint32_t nz) {
+    for (uint32_t z = blockIdx.z; z < nz; z += gridDim.z) {
+        for (uint32_t y = blockIdx.y; y < ny; y += gridDim.y) {
+            const uint64_t gy = (uint64_t)(y0 + y), gz = (uint64_t)(z0 + z);
+            const int32_t* s = src.vox + (src.origin + (uint64_t)x0 + src.SX * gy + src.SXY * gz);
+            int32_t* d = dst.vox + (dst.origin + (uint64_t)x0 + dst.SX * gy + dst.SXY * gz);
+            for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += gridDim.x * blockDim.x) d[x] = s[x];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_fill(int32_t* p, uint64_t n, int32_t v) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -357,6 +373,20 @@ int launch_copy(int32_t* dst, const int32_t* src, uint64_t n, hipStream_t s) {
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(k_copy, dim3((uint32_t)blocks), dim3(256), 0, s, (int4*)dst, (const int4*)src, n16,
                        dst + 4 * n16, src + 4 * n16, tail);
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_copy_box(const GridDesc& src, const GridDesc& dst, int64_t x0, int64_t y0, int64_t z0, int64_t nx,
+                    int64_t ny, int64_t nz, hipStream_t s) {
+    if (nx <= 0 || ny <= 0 || nz <= 0) return VRT_OK;
+    if (x0 < 0 || y0 < 0 || z0 < 0 || x0 + nx > src.X || y0 + ny > src.Y || z0 + nz > src.Z || src.X != dst.X ||
+        src.Y != dst.Y || src.Z != dst.Z)
+        return fail(VRT_ERR_INVALID, "copy box outside the grid");
+    const uint32_t bx = (uint32_t)std::min<int64_t>((nx + 255) / 256, 1024);
+    const uint32_t by = (uint32_t)std::min<int64_t>(ny, 65535), bz = (uint32_t)std::min<int64_t>(nz, 65535);
+    hipLaunchKernelGGL(k_copy_box, dim3(bx, by, bz), dim3(256), 0, s, src, dst, x0, y0, z0, (uint32_t)nx,
+                       (uint32_t)ny, (uint32_t)nz);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }

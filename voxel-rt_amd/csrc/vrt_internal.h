@@ -22,21 +22,42 @@ int hip_fail(hipError_t e, const char* what);
     } while (0)
 
 // Grid as the kernels see it.
+// Render copy of the grid: every side padded with kGuard cells holding
+// kGuardValue, the bits of a NaN.  A walk that steps onto one takes it for a
+// depth-field jump of NaN cells, which ends the walk at once as a miss, as
+// src/fshader.glsl:123-125 ends it on a cell outside the grid: the traversal
+// then needs no bounds test per step.  (A NaN jump inside the grid ends the
+// walk the same way in the reference, :114-121 then :83.)
+constexpr int32_t kGuard = 3;
+constexpr int32_t kGuardValue = (int32_t)0xFFC00000u;
+
+// Grid as the kernels see it.  Cell (x, y, z) is vox[origin + x + SX*y +
+// SXY*z]; X, Y, Z are the grid's extents.  The library's grid is dense
+// (SX = X, SXY = XY, origin 0); the render copy is padded (kGuard cells on
+// every side).
 struct GridDesc {
     int32_t* vox;
     uint32_t X, Y, Z;
     uint64_t XY;
-    uint64_t count;
+    uint64_t count;  // words allocated at vox
+    uint32_t SX;
+    uint64_t SXY;
+    uint64_t origin;
     // Device word: an upper bound on the y of every solid voxel (-1 when there
     // is none).  Every entry point that can add solid voxels raises it; edits
     // that only remove leave it high, which stays an upper bound.  Render
     // kernels stop a shadow ray once it is provably above it; null: they walk
     // every shadow ray to the reference's end.
     int32_t* top;
+    // The y and z a walk can index (the grid's and two cells past it on
+    // either side) fit signed 24-bit multiplies.
+    bool yz24() const { return (uint64_t)Y + 2 < (1ull << 23) && (uint64_t)Z + 2 < (1ull << 23); }
     // Index fits the 32-bit / 24-bit-multiply form (see Idx32).
-    bool small() const { return count <= 0xffffffffull && X < (1u << 23) && XY < (1ull << 23); }
+    bool small() const { return count <= 0xffffffffull && SX < (1u << 23) && SXY < (1ull << 23) && yz24(); }
     // Byte offset fits the 32-bit / 24-bit-multiply form (see IdxB).
-    bool bytes32() const { return count <= (1ull << 30) && 4ull * X < (1ull << 23) && 4ull * XY < (1ull << 23); }
+    bool bytes32() const {
+        return count <= (1ull << 30) && 4ull * SX < (1ull << 23) && 4ull * SXY < (1ull << 23) && yz24();
+    }
 };
 
 // One frame as the kernels see it: vrt_frame plus derived per-frame values.
@@ -123,6 +144,9 @@ int launch_set_voxel(const GridDesc& g, int64_t index, int32_t y, int32_t v, hip
 int launch_solid_top(const GridDesc& g, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1, int64_t z1,
                      bool reset, hipStream_t s);
 int launch_fill(int32_t* p, uint64_t n, int32_t v, hipStream_t s);
+// dst's cells of the box [x0,x0+nx) x [y0,y0+ny) x [z0,z0+nz) := src's.
+int launch_copy_box(const GridDesc& src, const GridDesc& dst, int64_t x0, int64_t y0, int64_t z0, int64_t nx,
+                    int64_t ny, int64_t nz, hipStream_t s);
 int launch_copy(int32_t* dst, const int32_t* src, uint64_t n, hipStream_t s);
 int launch_world_generate(const GridDesc& g, const vrt_world_params& p, const int32_t* cave_list,
                           int32_t n_caves, const int32_t* box_list, int32_t n_boxes,
@@ -150,6 +174,9 @@ struct vrt_ctx {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     vrt::GridDesc grid{};
+    // Padded render copy of `grid` (see kGuard); every entry point that
+    // writes the grid refreshes the part it wrote, on the same stream.
+    vrt::GridDesc rgrid{};
     vrt_frame frame{};
     bool frame_set = false;
     unsigned long long* counters = nullptr;  // rays, iters, pixels
@@ -168,13 +195,18 @@ struct vrt_ctx {
 };
 
 namespace vrt {
-// The grid as the render kernels get it: `top` only with the shadow cutoff on.
+// The grid as the render kernels get it: the padded copy, with `top` only
+// with the shadow cutoff on.
 inline GridDesc render_grid(const vrt_ctx* c) {
-    GridDesc g = c->grid;
-    if (!c->shadow_cutoff) g.top = nullptr;
+    GridDesc g = c->rgrid;
+    g.top = c->shadow_cutoff ? c->grid.top : nullptr;
     return g;
 }
 int solid_top_full(vrt_ctx* c);  // recomputes *grid.top over the whole grid
+// Copies the box [x0,x1) x [y0,y1) x [z0,z1) (clipped to the grid) of the
+// grid into the render copy, on the context's stream.
+int refresh_render(vrt_ctx* c, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1, int64_t z1);
+int refresh_render_all(vrt_ctx* c);
 }  // namespace vrt
 
 namespace vrt {

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
     dev::cast_ray<IX>(g, start[3 * i], start[3 * i + 1], start[3 * i + 2], dir[3 * i], dy, dir[3 * i + 2], lim, h,
                       (steps == nullptr && dy >= 0.0f) ? top : INT32_MAX);
     const bool hitp = h.index != -1;
-    if (hit) hit[i] = h.index;
+    if (hit) hit[i] = hitp ? IX::ref_index(g, h.index) : -1;
     if (hit_pos) {
         hit_pos[3 * i] = hitp ? h.pos[0] : 0.0f;
         hit_pos[3 * i + 1] = hitp ? h.pos[1] : 0.0f;
