@@ -252,16 +252,26 @@ __global__ void __launch_bounds__(256) k_copy(int4* __restrict__ dst, const int4
     if (t < tail) dst_tail[t] = src_tail[t];
 }
 
-// One x-row segment of the box per (blockIdx.y, blockIdx.z) step; the y and z
-// loops cover boxes taller or deeper than the launch.
+// One wave per x-row segment of the box (four rows per workgroup, the y and z
+// loops covering boxes taller or deeper than the launch); each lane moves the
+// words lane, lane+64, ... of its row, four loads in flight.
 __global__ void __launch_bounds__(256) k_copy_box(GridDesc src, GridDesc dst, int64_t x0, int64_t y0, int64_t z0,
                                                   uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (uint32_t z = blockIdx.z; z < nz; z += gridDim.z) {
-        for (uint32_t y = blockIdx.y; y < ny; y += gridDim.y) {
+        for (uint32_t y = blockIdx.y * 4 + w; y < ny; y += gridDim.y * 4) {
             const uint64_t gy = (uint64_t)(y0 + y), gz = (uint64_t)(z0 + z);
-            const int32_t* s = src.vox + (src.origin + (uint64_t)x0 + src.SX * gy + src.SXY * gz);
-            int32_t* d = dst.vox + (dst.origin + (uint64_t)x0 + dst.SX * gy + dst.SXY * gz);
-            for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += gridDim.x * blockDim.x) d[x] = s[x];
+            const int32_t* __restrict__ s = src.vox + (src.origin + (uint64_t)x0 + src.SX * gy + src.SXY * gz);
+            int32_t* __restrict__ d = dst.vox + (dst.origin + (uint64_t)x0 + dst.SX * gy + dst.SXY * gz);
+            uint32_t x = lane;
+            for (; x + 192 < nx; x += 256) {
+                const int32_t a = s[x], b = s[x + 64], c = s[x + 128], e = s[x + 192];
+                d[x] = a;
+                d[x + 64] = b;
+                d[x + 128] = c;
+                d[x + 192] = e;
+            }
+            for (; x < nx; x += 64) d[x] = s[x];
         }
     }
 }
@@ -383,9 +393,8 @@ int launch_copy_box(const GridDesc& src, const GridDesc& dst, int64_t x0, int64_
     if (x0 < 0 || y0 < 0 || z0 < 0 || x0 + nx > src.X || y0 + ny > src.Y || z0 + nz > src.Z || src.X != dst.X ||
         src.Y != dst.Y || src.Z != dst.Z)
         return fail(VRT_ERR_INVALID, "copy box outside the grid");
-    const uint32_t bx = (uint32_t)std::min<int64_t>((nx + 255) / 256, 1024);
-    const uint32_t by = (uint32_t)std::min<int64_t>(ny, 65535), bz = (uint32_t)std::min<int64_t>(nz, 65535);
-    hipLaunchKernelGGL(k_copy_box, dim3(bx, by, bz), dim3(256), 0, s, src, dst, x0, y0, z0, (uint32_t)nx,
+    const uint32_t by = (uint32_t)std::min<int64_t>((ny + 3) / 4, 65535), bz = (uint32_t)std::min<int64_t>(nz, 65535);
+    hipLaunchKernelGGL(k_copy_box, dim3(1, by, bz), dim3(256), 0, s, src, dst, x0, y0, z0, (uint32_t)nx,
                        (uint32_t)ny, (uint32_t)nz);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
