@@ -237,6 +237,8 @@ def batched_bench(vrt, reps=5):
     ctx.close()
     pm, bm = float(np.median(plain_ms)), float(np.median(batch_ms))
     done = np.median(np.array(plain_done), axis=0)
+    # algorithmic bytes of the set: 4 B per DDA iteration, RGB8 + f32 depth per pixel
+    alg = 4 * steps + 7 * pixels
     return {
         "workload": "64 views, resolutions uniform in {320x180,640x360,1280x720,1920x1080,3840x2160}, "
                     "random poses over a 1024x256x1024 grid (1 GiB), 0-16 lights per view",
@@ -244,6 +246,10 @@ def batched_bench(vrt, reps=5):
         "plain_ms": pm, "batched_ms": bm, "speedup": pm / bm,
         "plain_mrays_s": rays / (pm * 1e-3) / 1e6, "batched_mrays_s": rays / (bm * 1e-3) / 1e6,
         "frame_latency_ms": {"plain": percentiles(done), "batched": percentiles([bm] * len(views))},
+        "roofline": {"bound": "hbm", "bytes": int(alg), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "plain_achieved": alg / (pm * 1e-3) / 1e9, "plain_frac": alg / (pm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "batched_achieved": alg / (bm * 1e-3) / 1e9,
+                     "batched_frac": alg / (bm * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "tuned": {"tile": [best.tile_w, best.tile_h], "chunk_tiles": best.chunk_tiles, "tune_s": tune_s,
                   "sweep": [{"tile": [res[i].config.tile_w, res[i].config.tile_h],
                              "chunk": res[i].config.chunk_tiles, "ms": res[i].ms} for i in range(n.value)]},
@@ -306,6 +312,7 @@ def served_bench(vrt, n_requests=240, load=0.7):
         done += n.value
     wall = vrt.lib().vrt_server_clock(s) - t_start
     st = ctx.stats(reset=True)
+    alg = 4 * st["dda_steps"] + 7 * sum(w * h for (_, w, h) in views)
     vrt.call("vrt_server_destroy", s)
     # one removeSphere (carve + depth-field repair) on its own, device time
     # between events on the context's stream, radius 1-8 at surface points
@@ -325,6 +332,10 @@ def served_bench(vrt, n_requests=240, load=0.7):
                     % (n_requests, rate, 100 * load, edits),
         "latency_ms": percentiles(lat), "mrays_s": st["rays"] / wall / 1e6, "rays": int(st["rays"]),
         "wall_s": wall, "plain_ms_per_request": mean_ms,
+        "roofline": {"bound": "hbm", "bytes": int(alg), "achieved": alg / wall / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg / wall / 1e9 / HBM_PEAK_GBS,
+                     "note": "over the wall clock of the request stream, offered at %.0f%% of the plain rate"
+                             % (100 * load)},
         "remove_sphere_us": {"p50": float(np.percentile(edit_us, 50)), "p99": float(np.percentile(edit_us, 99)),
                              "sample": "40 edits, radius 1-8 at surface points, carve + repair"},
     }
@@ -443,11 +454,13 @@ def main():
     del sc
     if not args.no_extra and args.only != "plain":
         res["batched"] = batched_bench(vrt)
-        log("batched: plain %.2f ms, batched %.2f ms (x%.2f)" % (
-            res["batched"]["plain_ms"], res["batched"]["batched_ms"], res["batched"]["speedup"]))
+        log("batched: plain %.2f ms, batched %.2f ms (x%.2f); HBM roofline %.2f vs %.2f" % (
+            res["batched"]["plain_ms"], res["batched"]["batched_ms"], res["batched"]["speedup"],
+            res["batched"]["roofline"]["plain_frac"], res["batched"]["roofline"]["batched_frac"]))
         res["served"] = served_bench(vrt)
-        log("served: p50 %.2f ms p99 %.2f ms, %.0f Mrays/s" % (
-            res["served"]["latency_ms"]["p50"], res["served"]["latency_ms"]["p99"], res["served"]["mrays_s"]))
+        log("served: p50 %.2f ms p99 %.2f ms, %.0f Mrays/s, HBM roofline %.2f over the wall clock" % (
+            res["served"]["latency_ms"]["p50"], res["served"]["latency_ms"]["p99"], res["served"]["mrays_s"],
+            res["served"]["roofline"]["frac"]))
         res["large"] = large_bench(vrt)
         log("large: shaded %.2f ms (%.0f Mrays/s), depth view %.2f ms, depth field %.0f ms" % (
             res["large"]["shaded"]["kernel_ms"], res["large"]["shaded"]["mrays_s"],
