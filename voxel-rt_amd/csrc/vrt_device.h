@@ -123,8 +123,8 @@ struct Hit {
 // Clamps a cell coordinate to [-2, n + 1] (one v_med3_i32).  A walk indexes
 // the padded copy only through this: a cell two or more outside the grid
 // cannot enter it on the next step, and from -2 or n + 1 that step lands on a
-// guard cell (|kGuard| = 3) whatever its direction, as it lands outside the
-// grid from the true cell.
+// guard cell (kGuard = 3 of them on each side) whatever its direction, as it
+// lands outside the grid from the true cell.
 __device__ __forceinline__ int32_t clamp_cell(int32_t v, uint32_t n) {
     int32_t r;
     asm("v_med3_i32 %0, %1, -2, %2" : "=v"(r) : "v"(v), "s"((int32_t)n + 1));
