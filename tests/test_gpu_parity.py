@@ -424,6 +424,73 @@ def test_cast_rays_extreme_values_match_oracle(gpu, orc):
     ctx.close()
 
 
+def test_every_writer_keeps_the_render_copy_current(gpu, orc):
+    """The render kernels read a padded copy of the grid that each entry point
+    writing the grid refreshes.  After every kind of write, a frame rendered
+    on the GPU must equal the oracle's frame of the grid as downloaded."""
+    sc = gpu.make_scene("tiny", seed=5)
+    X, Y, Z = sc.dims
+    ctx = gpu.Context(sc.dims)
+    rng = np.random.default_rng(12)
+    hit_px = []
+
+    def check(label):
+        w = ctx.download()
+        g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
+        r = orc.render(w, sc.dims, sc.frame, sc.width, sc.height)
+        assert_frames_match(g, r, label)
+        hit_px.append(int((r["hit"] >= 0).sum()))
+        return r
+
+    check("fresh context (all -1)")
+    ctx.generate(sc.params)
+    r = check("device generator")
+    ctx.build_depth_field()
+    check("depth field")
+    # cells the camera sees: carve and place around a few hit voxels
+    hits = r["hit"][r["hit"] >= 0]
+    picks = rng.choice(hits, 6, replace=False)
+    cells = [(int(i % X), int((i // X) % Y), int(i // (X * Y))) for i in picks]
+    ctx.remove_sphere(*cells[0], 5)
+    check("removeSphere")
+    ctx.place_voxel(cells[1][0], cells[1][1] + 1, cells[1][2], 0x30A050)
+    check("placeVoxel")
+    ctx.destroy_voxel(*cells[2])
+    check("destroyVoxel")
+    lo = (max(cells[3][0] - 4, 0), max(cells[3][1] - 4, 0), max(cells[3][2] - 4, 0))
+    ctx.fix_depth_field_box(lo, (lo[0] + 9, lo[1] + 9, lo[2] + 9))
+    check("fixDepthField box")
+    w = ctx.download().reshape(Z, Y, X)
+    x, y, z = cells[4]
+    w[max(z - 3, 0):z + 3, y + 1:y + 4, max(x - 3, 0):x + 3] = 0xC08040
+    # end corner inside the grid, or the reference's corner rule would swap the corners
+    ctx.upload_region(w.reshape(-1), (max(x - 3, 0), y + 1, max(z - 3, 0)),
+                      (min(x + 2, X - 1), min(y + 4, Y - 1), min(z + 3, Z - 1)))
+    check("region upload")
+    w = ctx.download()
+    w[rng.random(w.size) < 0.002] = 0x102030
+    ctx.upload(w)
+    check("full upload")
+    other = gpu.DeviceBuffer(w.nbytes)
+    w2 = w.copy()
+    w2[rng.random(w.size) < 0.002] = -1
+    other.upload(w2)
+    gpu.call("vrt_upload_device", ctx.h, other.addr)
+    check("device-to-device upload")
+    p = gpu.P()
+    gpu.call("vrt_grid_device_ptr", ctx.h, gpu.C.byref(p))
+    w3 = w2.copy()
+    w3[rng.random(w.size) < 0.002] = 0x405060
+    buf = gpu.DeviceBuffer(w3.nbytes)
+    buf.upload(w3)
+    gpu.hip_check(gpu.hip().hipMemcpy(p, buf.ptr, buf.nbytes, gpu.D2D), "hipMemcpy D2D")
+    gpu.hip_check(gpu.hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+    ctx.grid_modified()
+    check("device pointer + vrt_grid_modified")
+    ctx.close()
+    assert hit_px[0] == 0 and min(hit_px[1:]) > 1000, hit_px
+
+
 def test_device_copy_and_generator(gpu):
     """vrt_upload_device copies a whole grid on the device; the device world
     generator writes the same words as the host generator."""
