@@ -110,6 +110,7 @@ FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target) {
     d.inv_w = width > 0 ? 1.0f / (float)width : 0.0f;
     d.inv_h = height > 0 ? 1.0f / (float)height : 0.0f;
     d.primary_cutoff = (f.view_depth_field != 1 && target.steps == nullptr) ? 1 : 0;
+    d.row_rot = 0;
     for (int i = 0; i < VRT_MAX_LOCAL_LIGHTS; i++) {
         const vrt_local_light& L = f.lights[i];
         d.light_dd_max[i] = sqrt_le_limit(L.radius);
@@ -118,6 +119,36 @@ FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target) {
         if (L.pos[0] >= 0.0f && L.pos[1] >= 0.0f && L.pos[2] >= 0.0f) d.light_valid |= 1u << i;
     }
     return d;
+}
+
+// First tile row of a plain draw.  A workgroup's time is set by its longest
+// walk, and the hardware deals workgroups in order, so the frame ends when
+// the last-dealt rows' walks end: the cheap rows belong last.  With primary
+// rays stopping above the grid's solid top, a row whose rays all rise (at the
+// left edge, centre and right edge of the frame) steeply enough to clear the
+// top within kCheapWalk cells is cheap; such rows at the top of the frame
+// (the sky of a camera without roll) are dealt after all the others.  Any
+// other case keeps row-major order.  Only the order changes, never a result.
+int32_t plain_row_rotation(const vrt_frame& f, const vrt_target& t, int32_t top, bool primary_cutoff) {
+    if (!primary_cutoff || top < 0 || t.height <= 0) return 0;
+    constexpr double kCheapWalk = 96.0;
+    const int32_t tiles_y = (t.height + 7) / 8;
+    const double rise = (double)top + 1.0 - (double)f.cam_pos[1];
+    const float* M = f.rotate_matrix;
+    int32_t r = 0;
+    for (; r < tiles_y; r++) {
+        const int32_t py = std::min(8 * r + 4, t.height - 1);
+        const double vy = (2.0 * (t.height - 1 - py) + 1.0) / t.height - 1.0;
+        double min_dy = INFINITY;
+        for (double vx : {-1.0, 0.0, 1.0}) {
+            const double ux = vx * f.aspect_ratio, uy = vy, uz = f.focal;
+            const double n = std::sqrt(ux * ux + uy * uy + uz * uz);
+            min_dy = std::min(min_dy, (M[1] * ux + M[5] * uy + M[9] * uz) / n);
+        }
+        const bool cheap = min_dy > 0.0 && (rise <= 0.0 || rise < kCheapWalk * min_dy);
+        if (!cheap) break;
+    }
+    return r == tiles_y ? 0 : r;
 }
 
 int check_frame(const vrt_frame& f) {
@@ -255,6 +286,13 @@ const float kIdentity[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
 
 }  // namespace
 
+// Reads *grid.top into top_host (waits for the context's stream).
+int read_top(vrt_ctx* c) {
+    VRT_HIP(hipMemcpyAsync(&c->top_host, c->grid.top, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    VRT_HIP(hipStreamSynchronize(c->stream));
+    return VRT_OK;
+}
+
 int solid_top_full(vrt_ctx* c) {
     return launch_solid_top(c->grid, 0, 0, 0, c->grid.X, c->grid.Y, c->grid.Z, true, c->stream);
 }
@@ -359,6 +397,7 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
     if (rc) return bail(rc);
     rc = launch_solid_top(c->grid, 0, 0, 0, 0, 0, 0, true, c->stream);  // no solid voxel yet: -1
     if (rc) return bail(rc);
+    c->top_host = -1;
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(hip_fail(e, "hipStreamSynchronize"));
     vrt_frame_defaults(&c->frame, X, 800, 600);
     *out = c;
@@ -408,8 +447,7 @@ int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels) {
                            ctx->stream));
     if (int rc = solid_top_full(ctx)) return rc;
     if (int rc = refresh_render_all(ctx)) return rc;
-    VRT_HIP(hipStreamSynchronize(ctx->stream));
-    return VRT_OK;
+    return read_top(ctx);
 }
 
 int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels) {
@@ -417,13 +455,15 @@ int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels) {
     if (!device_voxels) return fail(VRT_ERR_INVALID, "null voxels");
     if (int rc = launch_copy(ctx->grid.vox, device_voxels, ctx->grid.count, ctx->stream)) return rc;
     if (int rc = solid_top_full(ctx)) return rc;
-    return refresh_render_all(ctx);
+    if (int rc = refresh_render_all(ctx)) return rc;
+    return read_top(ctx);
 }
 
 int vrt_grid_modified(vrt_ctx* ctx) {
     if (int rc = check_ctx(ctx)) return rc;
     if (int rc = solid_top_full(ctx)) return rc;
-    return refresh_render_all(ctx);
+    if (int rc = refresh_render_all(ctx)) return rc;
+    return read_top(ctx);
 }
 
 int vrt_set_shadow_cutoff(vrt_ctx* ctx, int32_t enable) {
@@ -468,8 +508,7 @@ int vrt_upload_region(vrt_ctx* ctx, const int32_t* host_voxels, int32_t x0, int3
     // the region can only have gained solid voxels above the old bound
     if (int rc = launch_solid_top(g, xs, ys, zs, xe + 1, ye, ze, false, ctx->stream)) return rc;
     if (int rc = refresh_render(ctx, xs, ys, zs, xe + 1, ye, ze)) return rc;
-    VRT_HIP(hipStreamSynchronize(ctx->stream));
-    return VRT_OK;
+    return read_top(ctx);
 }
 
 int vrt_download(vrt_ctx* ctx, int32_t* host_voxels) {
@@ -546,6 +585,7 @@ int vrt_place_voxel(vrt_ctx* ctx, int32_t x, int32_t y, int32_t z, int32_t voxel
     if (x < 0 || y < 0 || z < 0 || (uint32_t)x >= g.X || (uint32_t)y >= g.Y || (uint32_t)z >= g.Z) return VRT_OK;
     if (int rc = launch_set_voxel(g, (int64_t)x + (int64_t)g.X * y + (int64_t)g.XY * z, y, voxel, ctx->stream))
         return rc;
+    if (voxel >= 0) ctx->top_host = std::max(ctx->top_host, y);  // as k_set_voxel raises *top
     return refresh_render(ctx, x, y, z, (int64_t)x + 1, (int64_t)y + 1, (int64_t)z + 1);
 }
 
@@ -652,7 +692,9 @@ int vrt_draw_frame(vrt_ctx* ctx, const vrt_frame* f, const vrt_target* t) {
     if (int rc = check_frame(*f)) return rc;
     if (int rc = check_target(*t)) return rc;
     ctx->draws++;
-    return launch_render_plain(render_grid(ctx), make_frame_desc(*f, *t), *t, ctx->counters, ctx->stream);
+    FrameDesc F = make_frame_desc(*f, *t);
+    if (ctx->shadow_cutoff) F.row_rot = plain_row_rotation(*f, *t, ctx->top_host, F.primary_cutoff != 0);
+    return launch_render_plain(render_grid(ctx), F, *t, ctx->counters, ctx->stream);
 }
 
 // glDrawArrays over the fullscreen quad, src/main.cpp:59.
@@ -737,7 +779,8 @@ int vrt_world_generate_device(vrt_ctx* ctx, const vrt_world_params* p) {
     if (!p) return fail(VRT_ERR_INVALID, "null params");
     if (int rc = world_generate_device(ctx->grid, *p, ctx->stream)) return rc;
     if (int rc = solid_top_full(ctx)) return rc;
-    return refresh_render_all(ctx);
+    if (int rc = refresh_render_all(ctx)) return rc;
+    return read_top(ctx);
 }
 
 }  // extern "C"

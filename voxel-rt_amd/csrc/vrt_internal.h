@@ -21,7 +21,6 @@ int hip_fail(hipError_t e, const char* what);
         if (e_ != hipSuccess) return vrt::hip_fail(e_, #call); \
     } while (0)
 
-// Grid as the kernels see it.
 // Render copy of the grid: every side padded with kGuard cells holding
 // kGuardValue, the bits of a NaN.  A walk that steps onto one takes it for a
 // depth-field jump of NaN cells, which ends the walk at once as a miss, as
@@ -79,6 +78,9 @@ struct FrameDesc {
     // Primary rays may stop above every solid voxel too: only when nothing
     // shows their step count (no depth view, no step map in the target).
     int32_t primary_cutoff;
+    // k_render_plain deals tile rows from this one on, wrapping to row 0
+    // (dispatch order only; see plain_row_rotation).
+    int32_t row_rot;
 };
 
 // Ray/step counters: kCounterSlots slots of one 128-byte line each
@@ -192,6 +194,9 @@ struct vrt_ctx {
     bool tuned = false;
     vrt_batch_config tuned_cfg{8, 8, 0, 0};
     bool shadow_cutoff = true;  // vrt_set_shadow_cutoff
+    // Host copy of *grid.top, read back by the entry points that can raise it
+    // (they synchronize anyway or are not on a per-frame path).
+    int32_t top_host = -1;
 };
 
 namespace vrt {
@@ -212,6 +217,7 @@ int refresh_render_all(vrt_ctx* c);
 namespace vrt {
 int world_generate_device(const GridDesc& g, const vrt_world_params& p, hipStream_t s);
 FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target);
+int32_t plain_row_rotation(const vrt_frame& f, const vrt_target& t, int32_t top, bool primary_cutoff);
 int check_frame(const vrt_frame& f);
 int check_target(const vrt_target& t);
 }  // namespace vrt

@@ -67,8 +67,10 @@ k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counte
     const int32_t lane = threadIdx.x & 63;
     unsigned long long rays = 0, iters = 0, pixels = 0;
     if (tile < tiles_x * tiles_y) {
+        int32_t trow = tile / tiles_x + F.row_rot;
+        if (trow >= tiles_y) trow -= tiles_y;
         const int32_t px = (tile % tiles_x) * 8 + (lane & 7);
-        const int32_t row = (tile / tiles_x) * 8 + (lane >> 3);
+        const int32_t row = trow * 8 + (lane >> 3);
         if (px < t.width && row < t.height) {
             float vx, vy;
             dev::pixel_vpos(t.width, t.height, F.inv_w, F.inv_h, px, row, vx, vy);
