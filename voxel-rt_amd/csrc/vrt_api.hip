@@ -123,17 +123,21 @@ FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target) {
 
 // First tile row of a plain draw.  A workgroup's time is set by its longest
 // walk, and the hardware deals workgroups in order, so the frame ends when
-// the last-dealt rows' walks end: the cheap rows belong last.  With primary
-// rays stopping above the grid's solid top, a row whose rays all rise (at the
-// left edge, centre and right edge of the frame) steeply enough to clear the
-// top within kCheapWalk cells is cheap; such rows at the top of the frame
-// (the sky of a camera without roll) are dealt after all the others.  Any
-// other case keeps row-major order.  Only the order changes, never a result.
+// the last-dealt rows' walks end: rows of moderate cost belong last.  With
+// primary rays stopping above the grid's solid top and the camera below that
+// top, a row whose rays all rise (at the left edge, centre and right edge of
+// the frame) steeply enough to clear the top within kCheapWalk cells is such
+// a row; those at the top of the frame (the sky of a camera without roll) are
+// dealt after all the others (-3% to -4% on a 1080p frame whose camera is 5
+// cells under the top).  A camera above the top keeps row-major order: its
+// sky rays end after one jump, and dealing those near-empty workgroups last
+// made a 4K frame 30% slower.  Only the order changes, never a result.
 int32_t plain_row_rotation(const vrt_frame& f, const vrt_target& t, int32_t top, bool primary_cutoff) {
     if (!primary_cutoff || top < 0 || t.height <= 0) return 0;
     constexpr double kCheapWalk = 96.0;
     const int32_t tiles_y = (t.height + 7) / 8;
     const double rise = (double)top + 1.0 - (double)f.cam_pos[1];
+    if (!(rise > 0.0)) return 0;
     const float* M = f.rotate_matrix;
     int32_t r = 0;
     for (; r < tiles_y; r++) {
@@ -145,7 +149,7 @@ int32_t plain_row_rotation(const vrt_frame& f, const vrt_target& t, int32_t top,
             const double n = std::sqrt(ux * ux + uy * uy + uz * uz);
             min_dy = std::min(min_dy, (M[1] * ux + M[5] * uy + M[9] * uz) / n);
         }
-        const bool cheap = min_dy > 0.0 && (rise <= 0.0 || rise < kCheapWalk * min_dy);
+        const bool cheap = min_dy > 0.0 && rise < kCheapWalk * min_dy;
         if (!cheap) break;
     }
     return r == tiles_y ? 0 : r;
