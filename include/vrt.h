@@ -122,8 +122,9 @@ int vrt_sync(vrt_ctx* ctx);
 int vrt_grid_device_ptr(vrt_ctx* ctx, int32_t** out);
 /* Re-derives what the context keeps about the grid's contents after writes
  * made through vrt_grid_device_ptr: the highest y of any solid voxel (used by
- * the shadow cutoff) and the render kernels' padded copy of the grid.  The
- * library's own uploads and edits keep both current.  No counterpart in the
+ * the shadow cutoff and the plain draw's tile order) and the render kernels'
+ * padded copy of the grid.  The library's own uploads and edits keep both
+ * current.  Returns once the context's stream is idle.  No counterpart in the
  * reference, whose grid has no device-side writers. */
 int vrt_grid_modified(vrt_ctx* ctx);
 /* Cutoff (on by default): a shadow ray that is above every solid voxel of the
@@ -141,7 +142,10 @@ int vrt_grid_dims(vrt_ctx* ctx, int64_t dims[3]);
 /* ---- uploads: updateGeometry / updatePartialGeometry -------------------- */
 
 /* Full grid host->device; updateGeometry, src/render.cpp:199-202 (and the
- * initial glBufferData, src/render.cpp:366-369). */
+ * initial glBufferData, src/render.cpp:366-369).  This and the other calls
+ * that can add solid voxels in bulk (vrt_upload_device, vrt_upload_region,
+ * vrt_grid_modified, vrt_world_generate_device) return once the context's
+ * stream is idle: they read back the grid's highest solid y. */
 int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels);
 /* Full grid device->device (grid built or kept elsewhere on the GPU). */
 int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels);
