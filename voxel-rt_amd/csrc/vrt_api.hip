@@ -128,7 +128,7 @@ FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target) {
 // top, a row whose rays all rise (at the left edge, centre and right edge of
 // the frame) steeply enough to clear the top within kCheapWalk cells is such
 // a row; those at the top of the frame (the sky of a camera without roll) are
-// dealt after all the others (-3% to -4% on a 1080p frame whose camera is 5
+// dealt after all the others (-4% to -6% on a 1080p frame whose camera is 5
 // cells under the top).  A camera above the top keeps row-major order: its
 // sky rays end after one jump, and dealing those near-empty workgroups last
 // made a 4K frame 30% slower.  Only the order changes, never a result.
