@@ -88,11 +88,15 @@ __device__ __forceinline__ float div_small_int(float a, float d, float y) {
 // RN(1/d): v_rcp_f32 (within one ulp) refined by one Newton step with FMAs,
 // e = RN(1 - d*y0), y = RN(y0 + y0*e).  For |d| in [2^-60, 2^60] this equals
 // IEEE division 1/d for every input: the library's reciprocal selftest
-// compares the two for all 2^32 bit patterns.  Other inputs take the
-// division itself.
-__device__ __forceinline__ float rcp_rn(float d) {
+// compares the two for all 2^32 bit patterns.  rcp_fast is that form alone,
+// for callers that test the range themselves; rcp_rn gives any other input
+// the division itself.
+__device__ __forceinline__ float rcp_fast(float d) {
     const float y0 = __builtin_amdgcn_rcpf(d);
-    float y = __builtin_fmaf(__builtin_fmaf(-d, y0, 1.0f), y0, y0);
+    return __builtin_fmaf(__builtin_fmaf(-d, y0, 1.0f), y0, y0);
+}
+__device__ __forceinline__ float rcp_rn(float d) {
+    float y = rcp_fast(d);
     if (__builtin_expect(!div_range_ok(d), 0)) y = 1.0f / d;
     return y;
 }
@@ -233,11 +237,31 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
     const int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
     const int32_t stx = isign(dx), sty = isign(dy), stz = isign(dz);
     const int32_t fx = stx > 0, fy = sty > 0, fz = stz > 0;
-    const float ddx = rcp_rn(fabsf(dx + 0.000001f));
-    const float ddy = rcp_rn(fabsf(dy + 0.000001f));
-    const float ddz = rcp_rn(fabsf(dz + 0.000001f));
-    const float rx = rcp_rn(dx), ry = rcp_rn(dy), rz = rcp_rn(dz);
-    const bool dok = (int(div_range_ok(dx)) & int(div_range_ok(dy)) & int(div_range_ok(dz))) != 0;
+    // The six reciprocals (the step sizes 1/|d + 1e-6| of :74-76 and 1/d for
+    // the intersect quotients) share one range test: every |value| in
+    // [2^-60, 2^60] takes rcp_rn's fast form, anything else (a zero, a
+    // subnormal, a huge or infinite component) IEEE division for all six and
+    // for the quotients.  A NaN component is NaN on either path.
+    const float bx_ = fabsf(dx + 0.000001f), by_ = fabsf(dy + 0.000001f), bz_ = fabsf(dz + 0.000001f);
+    const float lo = fminf(fminf(fminf(fabsf(dx), fabsf(dy)), fabsf(dz)), fminf(fminf(bx_, by_), bz_));
+    const float hi = fmaxf(fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)), fmaxf(fmaxf(bx_, by_), bz_));
+    const bool dok = (lo >= 0x1p-60f) & (hi <= 0x1p60f);
+    float ddx, ddy, ddz, rx, ry, rz;
+    if (__builtin_expect(dok, 1)) {
+        ddx = rcp_fast(bx_);
+        ddy = rcp_fast(by_);
+        ddz = rcp_fast(bz_);
+        rx = rcp_fast(dx);
+        ry = rcp_fast(dy);
+        rz = rcp_fast(dz);
+    } else {
+        ddx = 1.0f / bx_;
+        ddy = 1.0f / by_;
+        ddz = 1.0f / bz_;
+        rx = 1.0f / dx;
+        ry = 1.0f / dy;
+        rz = 1.0f / dz;
+    }
     float ix, iy, iz;
     div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok, ix, iy,
             iz);
