@@ -424,6 +424,43 @@ def test_cast_rays_extreme_values_match_oracle(gpu, orc):
     ctx.close()
 
 
+def test_random_views_sweep(gpu, orc):
+    """Random cameras over two worlds: low and high above the terrain (below
+    and above the grid's solid top), pitched up and down, with and without a
+    step map, in odd frame sizes.  Plain draws deal their tile rows in a
+    camera-dependent order and walk the padded render copy; every map must
+    equal the oracle's."""
+    rng = np.random.default_rng(2024)
+    hits = []
+    for name, seed in (("tiny", 9), ("c1", 6)):
+        sc = gpu.make_scene(name, seed=seed)
+        X, Y, Z = sc.dims
+        ctx = gpu.Context(sc.dims)
+        ctx.upload(sc.world)
+        ctx.build_depth_field()
+        wdf = ctx.download()
+        top = int(np.nonzero((sc.world.reshape(Z, Y, X) >= 0).any(axis=(0, 2)))[0].max())
+        for k in range(8):
+            w, h = int(rng.integers(40, 200)), int(rng.integers(30, 150))
+            high = k % 2 == 1
+            ch = (top - sc.params.height_min + 2, top - sc.params.height_min + 12) if high else (2, 10)
+            # cameras above the top look down (positive pitch), the others either way
+            f = gpu.make_frame(sc.params, sc.dims, sc.world, rng, w, h, int(rng.integers(0, 17)), 384,
+                               pitch_range=(0.15, 0.6) if high else (-0.6, 0.6), cam_height=ch, light_spread=40.0)
+            r = orc.render(wdf, sc.dims, f, w, h)
+            for steps in (False, True):
+                t = gpu.Target(w, h, depth=True, aux=True, steps=steps)
+                ctx.stats(reset=True)
+                ctx.draw(t, frame=f)
+                g = t.download()
+                g.update(ctx.stats(reset=True))
+                assert_frames_match(g, r, "%s view %d (%dx%d, cam y %.1f, top %d) steps=%d" % (
+                    name, k, w, h, f.cam_pos[1], top, steps))
+            hits.append(int((r["hit"] >= 0).sum()))
+        ctx.close()
+    assert sum(1 for v in hits if v > 100) >= 12, hits
+
+
 def test_every_writer_keeps_the_render_copy_current(gpu, orc):
     """The render kernels read a padded copy of the grid that each entry point
     writing the grid refreshes.  After every kind of write, a frame rendered
