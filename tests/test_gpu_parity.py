@@ -424,6 +424,47 @@ def test_cast_rays_extreme_values_match_oracle(gpu, orc):
     ctx.close()
 
 
+def test_cast_rays_fuzz_small_and_odd_grids(gpu, orc):
+    """castRay on 24 grids of random size (1 to 37 cells per axis, so the
+    guard cells and the clamp meet one- and two-cell extents), random solid
+    density, depth field built; rays from inside, near and far outside, with
+    zero, tiny, unnormalized and axis-aligned directions and random budgets."""
+    rng = np.random.default_rng(4321)
+    total_hits = 0
+    for case in range(24):
+        dims = tuple(int(v) for v in rng.integers(1, 38, 3))
+        X, Y, Z = dims
+        n = X * Y * Z
+        world = np.where(rng.random(n) < rng.uniform(0.0, 0.4), rng.integers(0, 1 << 24, n), -1).astype(np.int32)
+        ctx = gpu.Context(dims)
+        ctx.upload(world)
+        ctx.build_depth_field()
+        wdf = ctx.download()
+        m = 3000
+        start = np.stack([rng.uniform(-6.0, dims[k] + 6.0, m) for k in range(3)], 1)
+        d = rng.normal(size=(m, 3))
+        kind = rng.integers(0, 6, m)
+        d[kind == 1] *= 60.0                                       # unnormalized, long jumps
+        d[kind == 2, rng.integers(0, 3)] = 0.0                     # a zero component
+        d[kind == 3] = np.eye(3)[rng.integers(0, 3, int((kind == 3).sum()))] * rng.choice([-1, 1], (int((kind == 3).sum()), 1))
+        d[kind == 4, rng.integers(0, 3)] = 1e-7                    # a tiny component
+        start[kind == 5] = np.floor(start[kind == 5])              # integer starts (cell faces)
+        start, d = start.astype(np.float32), d.astype(np.float32)
+        dist = rng.integers(0, 300, m).astype(np.int32)
+        for steps in (True, False):
+            g = ctx.cast_rays(start, d, dist, 256, steps=steps)
+            r = orc.cast_rays(wdf, dims, start, d, dist, 256)
+            keys = ("hit", "face", "steps") if steps else ("hit", "face")
+            for k in keys:
+                bad = np.nonzero(g[k] != r[k])[0]
+                assert bad.size == 0, "grid %s: %s differs for %d rays, first %d (start %s dir %s)" % (
+                    dims, k, bad.size, bad[0], start[bad[0]].tolist(), d[bad[0]].tolist())
+            assert same_floats(g["hit_pos"], r["hit_pos"]), "grid %s hit_pos" % (dims,)
+        total_hits += int((r["hit"] >= 0).sum())
+        ctx.close()
+    assert total_hits > 5000, total_hits
+
+
 def test_random_views_sweep(gpu, orc):
     """Random cameras over two worlds: low and high above the terrain (below
     and above the grid's solid top), pitched up and down, with and without a
