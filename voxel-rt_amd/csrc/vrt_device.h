@@ -297,16 +297,16 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             // the jump recomputes every intersect, so the step's update is skipped
             const float toJump = -__int_as_float(v);
             trav += toJump;
-            // A jump that leaves `travelled` at or past the budget (or NaN:
-            // a guard cell) ends the walk at the loop test (:83); nothing the
-            // jump recomputes is read after that.
-            if (!(trav < lim)) break;
             const float c2 = cur + toJump;
             sx = dx * c2 + sx;
             sy = dy * c2 + sy;
             sz = dz * c2 + sz;
             const int32_t cx = f2i(sx), cy = f2i(sy), cz = f2i(sz);
-            if (cy > ytop) break;
+            // A jump that leaves `travelled` at or past the budget (or NaN:
+            // a guard cell) ends the walk at the loop test (:83); nothing the
+            // jump recomputes is read after that.  One branch for that and
+            // the cutoff.
+            if (!(trav < lim) | (cy > ytop)) break;
             div3_by((float)(cx + fx) - sx, (float)(cy + fy) - sy, (float)(cz + fz) - sz, dx, dy, dz, rx, ry, rz, dok,
                     ix, iy, iz);
             idx = IX::make(g, clamp_cell(cx, g.X), clamp_cell(cy, g.Y), clamp_cell(cz, g.Z));
