@@ -281,7 +281,8 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
         const bool bx = (ix < iy) & (ix < iz);
         const bool by = (iy < ix) & (iy < iz);
         const bool bz = !(bx | by);
-        idx += bx ? sdx : (by ? sdy : sdz);
+        const T dsel = bx ? sdx : (by ? sdy : sdz);
+        idx += dsel;
         const int32_t v = IX::load(vox, idx);
         // currentIntersect: the intersect of the axis just stepped, before its update
         const float cur = bx ? ix : (by ? iy : iz);
@@ -320,8 +321,10 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             // reference's own light orbit keeps lightPos.z = X/2: a sun ray
             // from a hit on the plane z = X/2 has dir.z = +0 and a NaN z
             // intersect, so z is always the axis taken, and it sticks here
-            // when its cell holds -1.)
-            if ((bx ? stx : (by ? sty : stz)) == 0) {
+            // when its cell holds -1.)  The stride just added is 0 exactly
+            // when that step is: a stride is the step times 1, SX or SXY
+            // (times 4 for byte offsets), nonzero in the index's width.
+            if (dsel == 0) {
                 while (trav < lim) {
                     iters++;
                     trav += 1.0f;
