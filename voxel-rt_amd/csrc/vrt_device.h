@@ -161,9 +161,11 @@ struct IdxB {
         asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(t) : "v"(z), "s"((int32_t)(g.SXY * 4)), "v"(t));
         return t;
     }
+    // stride of one step in words; advance() adds it as bytes
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
-        return 4u * (axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.SX : (uint32_t)sz * (uint32_t)g.SXY));
+        return axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.SX : (uint32_t)sz * (uint32_t)g.SXY);
     }
+    __device__ __forceinline__ static T advance(T i, T d) { return i + (d << 2); }
     __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) {
         return *(const int32_t*)((const char*)vox + i);
     }
@@ -183,6 +185,7 @@ struct Idx32 {
     __device__ __forceinline__ static T delta(const GridDesc& g, int32_t sx, int32_t sy, int32_t sz, int axis) {
         return axis == 0 ? (uint32_t)sx : (axis == 1 ? (uint32_t)sy * g.SX : (uint32_t)sz * (uint32_t)g.SXY);
     }
+    __device__ __forceinline__ static T advance(T i, T d) { return i + d; }
     __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) { return vox[i]; }
     __device__ __forceinline__ static int64_t word(T i) { return (int64_t)i; }
     __device__ __forceinline__ static int64_t ref_index(const GridDesc& g, int64_t w) {
@@ -199,6 +202,7 @@ struct Idx64 {
         return axis == 0 ? (uint64_t)(int64_t)sx
                          : (axis == 1 ? (uint64_t)(int64_t)sy * g.SX : (uint64_t)(int64_t)sz * g.SXY);
     }
+    __device__ __forceinline__ static T advance(T i, T d) { return i + d; }
     __device__ __forceinline__ static int32_t load(const int32_t* vox, T i) { return vox[i]; }
     __device__ __forceinline__ static int64_t word(T i) { return (int64_t)i; }
     __device__ __forceinline__ static int64_t ref_index(const GridDesc& g, int64_t w) {
@@ -282,7 +286,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
         const bool by = (iy < ix) & (iy < iz);
         const bool bz = !(bx | by);
         const T dsel = bx ? sdx : (by ? sdy : sdz);
-        idx += dsel;
+        idx = IX::advance(idx, dsel);
         const int32_t v = IX::load(vox, idx);
         // currentIntersect: the intersect of the axis just stepped, before its update
         const float cur = bx ? ix : (by ? iy : iz);
@@ -322,8 +326,8 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
             // from a hit on the plane z = X/2 has dir.z = +0 and a NaN z
             // intersect, so z is always the axis taken, and it sticks here
             // when its cell holds -1.)  The stride just added is 0 exactly
-            // when that step is: a stride is the step times 1, SX or SXY
-            // (times 4 for byte offsets), nonzero in the index's width.
+            // when that step is: a stride is the step times 1, SX or SXY,
+            // nonzero in the index's width.
             if (dsel == 0) {
                 while (trav < lim) {
                     iters++;
