@@ -54,7 +54,7 @@ def profiled(key):
     (SQ_THREAD_CYCLES_VALU, one per active lane per VALU instruction) or
     "valu_issue_frac" (4 x SQ_INSTS_VALU over 1024 SIMDs x the launch's
     cycles: the share of VALU issue slots used)."""
-    path = os.path.join(ROOT, "profiles", "k_render_plain_c2_1080p_counters.md")
+    path = os.path.join(ROOT, "profiles", "k_render_plain_1080p_16lights_counters.md")
     try:
         for line in open(path):
             if line.startswith(key + ":"):
@@ -72,13 +72,13 @@ def grid_ptr(vrt, ctx):
 
 
 def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
-    sc = vrt.make_scene("c2", seed=1)
+    sc = vrt.make_scene("frame1080", seed=1)
     ctx = vrt.Context(sc.dims)
     ctx.upload(sc.world)
     t0 = time.time()
     ctx.build_depth_field()
     ctx.sync()
-    log("c2 world %s, depth field %.3f s" % (sc.dims, time.time() - t0))
+    log("1080p world %s, depth field %.3f s" % (sc.dims, time.time() - t0))
     # One device-to-device copy of the whole grid (256 MiB read + 256 MiB
     # written by k_copy): a plain copy of known size that calibrates the
     # profiler's FETCH_SIZE / WRITE_SIZE counters for this run.
@@ -184,7 +184,7 @@ def batched_bench(vrt, reps=5):
     batch launch (autotuned tile shape and scheduling -- hardware-scheduled
     workgroups or persistent waves with a given queue chunk -- as a hipGraph)."""
     C = vrt.C
-    params, dims, world, ctx = world_ctx(vrt, "c4", 4)
+    params, dims, world, ctx = world_ctx(vrt, "views64", 4)
     views = vrt.make_views(params, dims, world, 44, 64)
     del world
     targets = [vrt.Target(w, h, depth=True) for (_, w, h) in views]
@@ -261,7 +261,7 @@ def served_bench(vrt, n_requests=240, load=0.7):
     in the batched set) at `load` x the plain throughput, 10 removeSphere edits
     (radius 1-8 at surface points) per 100 requests, 32 GB framebuffer budget."""
     C = vrt.C
-    params, dims, world, ctx = world_ctx(vrt, "c5", 5)
+    params, dims, world, ctx = world_ctx(vrt, "served2g", 5)
     X, Y, Z = dims
     views = vrt.make_views(params, dims, world, 55, n_requests)
     del world
@@ -346,7 +346,7 @@ def large_bench(vrt, frames=10):
     below the surface) generated on the device, its depth field, and 4K
     frames from a grazing camera with a 4096-step cap, shaded and in the
     depth-field view (the same scene the large-world parity test checks)."""
-    sc = vrt.make_scene("c3", seed=3)
+    sc = vrt.make_scene("large4k", seed=3)
     ctx = vrt.Context(sc.dims)
     t0 = time.perf_counter()
     ctx.generate(sc.params)

@@ -41,7 +41,7 @@ def same_outputs(a, b):
 
 @pytest.fixture(scope="module")
 def scene(gpu, orc):
-    sc = gpu.make_scene("c1", seed=21)
+    sc = gpu.make_scene("terrain128", seed=21)
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
     ctx.build_depth_field()
@@ -156,7 +156,7 @@ def drain(gpu, s, expect):
 
 
 def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
-    sc = gpu.make_scene("c1", seed=31)
+    sc = gpu.make_scene("terrain128", seed=31)
     X, Y, Z = sc.dims
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
@@ -203,7 +203,7 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
 
 
 def test_server_same_frame_alone_and_in_the_mix(gpu):
-    sc = gpu.make_scene("c1", seed=41)
+    sc = gpu.make_scene("terrain128", seed=41)
     views = small_views(gpu, sc, 10, seed=8)
     outs = []
     for mix in (False, True):

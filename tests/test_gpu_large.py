@@ -31,7 +31,7 @@ def compare_sample(g, r, pix, label):
 
 
 def test_large_world_4k_shaded_and_depth_view(gpu, orc):
-    sc = gpu.make_scene("c3", seed=3)
+    sc = gpu.make_scene("large4k", seed=3)
     X, Y, Z = sc.dims
     print("large world generated", flush=True)
     ctx = gpu.Context(sc.dims)

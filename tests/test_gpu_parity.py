@@ -40,7 +40,7 @@ def assert_frames_match(g, r, label=""):
         assert g["dda_steps"] == r["dda_steps"], "%s steps %d vs %d" % (label, g["dda_steps"], r["dda_steps"])
 
 
-@pytest.mark.parametrize("name", ["tiny", "c1"])
+@pytest.mark.parametrize("name", ["tiny", "terrain128"])
 def test_depth_field_build_matches_oracle(gpu, orc, name):
     sc = gpu.make_scene(name, seed=3)
     ctx = gpu.Context(sc.dims)
@@ -52,8 +52,8 @@ def test_depth_field_build_matches_oracle(gpu, orc, name):
     ctx.close()
 
 
-@pytest.mark.parametrize("name,seed,mode", [("tiny", 1, 0), ("tiny", 2, 1), ("c1", 1, 0), ("c1", 4, 0),
-                                            ("c1", 5, 1)])
+@pytest.mark.parametrize("name,seed,mode", [("tiny", 1, 0), ("tiny", 2, 1), ("terrain128", 1, 0),
+                                            ("terrain128", 4, 0), ("terrain128", 5, 1)])
 def test_frame_matches_oracle(gpu, orc, name, seed, mode):
     print("frame case", name, seed, mode, flush=True)
     sc = gpu.make_scene(name, seed=seed, depth_view=mode)
@@ -64,7 +64,7 @@ def test_frame_matches_oracle(gpu, orc, name, seed, mode):
     assert (r["hit"] >= 0).mean() > 0.2  # the frame sees terrain
 
 
-@pytest.mark.parametrize("name,seed", [("tiny", 1), ("c1", 4)])
+@pytest.mark.parametrize("name,seed", [("tiny", 1), ("terrain128", 4)])
 def test_shadow_cutoff_changes_only_the_step_total(gpu, orc, name, seed):
     """Every output is the same with the shadow cutoff on and off; off, the
     DDA-step total is the oracle's; on, it is smaller (sun rays rise above
@@ -173,7 +173,7 @@ def test_cast_rays_match_oracle(gpu, orc):
 
 
 def test_remove_sphere_sequence_matches_oracle(gpu, orc):
-    sc = gpu.make_scene("c1", seed=9)
+    sc = gpu.make_scene("terrain128", seed=9)
     X, Y, Z = sc.dims
     ref = orc.depth_field(sc.world, sc.dims)
     ctx = gpu.Context(sc.dims)
@@ -220,10 +220,10 @@ def test_place_destroy_and_region_upload(gpu, orc):
     ctx.close()
 
 
-def test_c2_frame_matches_oracle(gpu, orc):
+def test_frame1080_matches_oracle(gpu, orc):
     """The 1080p drop-in frame: GPU-built depth field checked on a sample of
     voxels, then the whole frame against the oracle on that world."""
-    sc = gpu.make_scene("c2", seed=1)
+    sc = gpu.make_scene("frame1080", seed=1)
     X, Y, Z = sc.dims
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
@@ -238,14 +238,14 @@ def test_c2_frame_matches_oracle(gpu, orc):
     assert np.array_equal(wdf[idx], exp), "depth field differs on %d sampled voxels" % int((wdf[idx] != exp).sum())
     g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
     r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height)
-    assert_frames_match(g, r, "c2")
+    assert_frames_match(g, r, "frame1080")
     ctx.close()
 
 
-def test_c2_more_cameras_match_oracle(gpu, orc):
+def test_frame1080_more_cameras_match_oracle(gpu, orc):
     """Six more 1080p cameras over the C2 world (lights clustered around the
     camera so that many are in range, one depth-view frame), cutoff on."""
-    sc = gpu.make_scene("c2", seed=1)
+    sc = gpu.make_scene("frame1080", seed=1)
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
     ctx.build_depth_field()
@@ -260,17 +260,17 @@ def test_c2_more_cameras_match_oracle(gpu, orc):
         g = t.download()
         g.update(ctx.stats(reset=True))
         r = orc.render(wdf, sc.dims, f, 1920, 1080)
-        assert_frames_match(g, r, "c2 camera %d" % k)
+        assert_frames_match(g, r, "1080p camera %d" % k)
     ctx.close()
 
 
-@pytest.mark.parametrize("case", ["c2_cameras", "c1"])
+@pytest.mark.parametrize("case", ["frame1080_cameras", "terrain128"])
 def test_primary_cutoff_keeps_every_output(gpu, orc, case):
     """Without a step map (and outside the depth view) primary rays heading
     up stop once above every solid voxel, like shadow rays: every map the
     target has still equals the oracle's, the iteration total drops below the
     oracle's, and with a step map (or the cutoff off) nothing stops early."""
-    sc = gpu.make_scene("c2" if case == "c2_cameras" else "c1", seed=1 if case == "c2_cameras" else 4)
+    sc = gpu.make_scene("frame1080" if case == "frame1080_cameras" else "terrain128", seed=1 if case == "frame1080_cameras" else 4)
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
     ctx.build_depth_field()
@@ -473,7 +473,7 @@ def test_random_views_sweep(gpu, orc):
     equal the oracle's."""
     rng = np.random.default_rng(2024)
     hits = []
-    for name, seed in (("tiny", 9), ("c1", 6)):
+    for name, seed in (("tiny", 9), ("terrain128", 6)):
         sc = gpu.make_scene(name, seed=seed)
         X, Y, Z = sc.dims
         ctx = gpu.Context(sc.dims)
@@ -572,7 +572,7 @@ def test_every_writer_keeps_the_render_copy_current(gpu, orc):
 def test_device_copy_and_generator(gpu):
     """vrt_upload_device copies a whole grid on the device; the device world
     generator writes the same words as the host generator."""
-    for name, seed in (("tiny", 3), ("c1", 2)):
+    for name, seed in (("tiny", 3), ("terrain128", 2)):
         cfg = gpu.CONFIGS[name]
         hmin, hmax = cfg.get("heights", (16, 96))
         p = gpu.world_params(cfg["dims"], seed, caves=cfg["caves"], boxes=cfg["boxes"] + 50, height_min=hmin,

@@ -555,11 +555,11 @@ def surface(params, x, z):
 # of area, or the count giving ~35% empty cells below the surface for the
 # large world), plus 2000 boxes of 1-12 voxels for the 1080p drop-in world.
 CONFIGS = {
-    "c1": dict(dims=(128, 128, 128), caves=32, boxes=0, size=(256, 256), lights=0, render_dist=384),
-    "c2": dict(dims=(512, 256, 512), caves=512, boxes=2000, size=(1920, 1080), lights=16, render_dist=384),
-    "c3": dict(dims=(2048, 512, 2048), caves=70000, boxes=0, size=(3840, 2160), lights=16, render_dist=4096),
-    "c4": dict(dims=(1024, 256, 1024), caves=2048, boxes=0, size=None, lights=None, render_dist=384),
-    "c5": dict(dims=(1024, 512, 1024), caves=2048, boxes=0, size=None, lights=None, render_dist=384),
+    "terrain128": dict(dims=(128, 128, 128), caves=32, boxes=0, size=(256, 256), lights=0, render_dist=384),
+    "frame1080": dict(dims=(512, 256, 512), caves=512, boxes=2000, size=(1920, 1080), lights=16, render_dist=384),
+    "large4k": dict(dims=(2048, 512, 2048), caves=70000, boxes=0, size=(3840, 2160), lights=16, render_dist=4096),
+    "views64": dict(dims=(1024, 256, 1024), caves=2048, boxes=0, size=None, lights=None, render_dist=384),
+    "served2g": dict(dims=(1024, 512, 1024), caves=2048, boxes=0, size=None, lights=None, render_dist=384),
     # small scenes for smoke and quick parity runs
     "tiny": dict(dims=(96, 64, 96), caves=8, boxes=40, size=(160, 120), lights=4, render_dist=384, heights=(8, 40)),
 }
@@ -634,7 +634,7 @@ def make_scene(name, seed=1, world=None, depth_view=0):
         n_lights = 16
     else:
         (w, h), n_lights = cfg["size"], cfg["lights"]
-    pitch = (0.035, 0.175) if name == "c3" else (0.15, 0.5)
+    pitch = (0.035, 0.175) if name == "large4k" else (0.15, 0.5)
     frame = make_frame(params, dims, world, rng, w, h, n_lights, cfg["render_dist"], pitch_range=pitch,
                        depth_view=depth_view)
     return Scene(name, dims, params, world, frame, w, h)
