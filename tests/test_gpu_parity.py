@@ -682,8 +682,8 @@ def test_pin_rays_match_oracle(gpu, orc):
         ctx.close()
 
 
-@pytest.mark.parametrize("case", ["range", "overbright", "falloff27", "falloff28", "roof", "clamp0", "clamp15",
-                                  "clamp14_15"])
+@pytest.mark.parametrize("case", ["range", "axis_range", "overbright", "falloff27", "falloff28", "roof", "clamp0",
+                                  "clamp15", "clamp14_15"])
 def test_pin_frames_match_oracle(gpu, orc, case):
     """The exact local-light scenes that pin the oracle (light at exactly its
     radius, overbright reached at exactly 1.25, linear falloff, shadow rays
@@ -696,6 +696,7 @@ def test_pin_frames_match_oracle(gpu, orc, case):
     else:
         dims, w = pins.shading_world()
         f = {"range": lambda: pins.light_range_frame(gpu, dims),
+             "axis_range": lambda: pins.axis_range_frame(gpu, dims),
              "overbright": lambda: pins.overbright_frame(gpu, dims),
              "falloff27": lambda: pins.falloff_frame(gpu, dims, float(2 ** 27)),
              "falloff28": lambda: pins.falloff_frame(gpu, dims, float(2 ** 28)),

@@ -449,6 +449,32 @@ def test_light_at_exactly_its_radius_is_evaluated(orc, vrt):
     assert np.all(r["rgb"].reshape(-1, 3) == [lit(c, m) for c in (200, 100, 50)])
 
 
+def axis_range_frame(vrt, dims):
+    """Sun overhead; two lights 2^26 cells along +x from the hit points (x =
+    2^26 + 16: every hit x near 16 rounds the x difference to exactly 2^26,
+    and the y and z differences vanish beside 2^52 in the squared length), of
+    radius 2^26 (in range) and 2^26 - 8 (out): the range decision rests on
+    the x difference alone, at its boundary."""
+    f = down_frame(vrt, dims, [16.0, HIGH, 16.0])
+    for i, radius in ((0, float(2 ** 26)), (1, float(2 ** 26 - 8))):
+        L = f.lights[i]
+        L.pos[:] = [float(2 ** 26 + 16), 6.5, 16.0]
+        L.radius = radius
+        L.rgb[:] = [1.0, 1.0, 1.0]
+    return f
+
+
+def test_light_range_decided_exactly_along_x(orc, vrt):
+    """fshader.glsl:171: length <= radius with the length carried by one axis."""
+    dims, w = shading_world()
+    wdf = orc.depth_field(w, dims)
+    r = orc.render(wdf, dims, axis_range_frame(vrt, dims), 8, 8)
+    assert np.all(r["hit"] >= 0)
+    assert np.all((r["shadow_eval"] & 0b110) == 0b010)  # light 0 evaluated, light 1 not
+    m = np.float32(0.4) + np.float32(0.8)  # light 0 adds N.L * 0 (falloff 0 at d == r)
+    assert np.all(r["rgb"].reshape(-1, 3) == [lit(c, m) for c in (200, 100, 50)])
+
+
 def overbright_frame(vrt, dims):
     """Sun overhead gives 0.4 + 0.8 = 1.2 and light 0 (falloff 1, since
     2^60 - 2^26 rounds to 2^60) adds 0.05: float32 rounds 1.2 + 0.05 to
