@@ -56,15 +56,20 @@
 
 namespace {
 
+/* The grid as the shader's storage buffer: word i is at(i).  The functions
+ * below take any type with X, Y, Z and at() (the lazily evaluated world
+ * further down is the other one). */
 struct Grid {
     const int32_t* vox;
     int64_t X, Y, Z;
+    int32_t at(int64_t i) const { return vox[i]; }
 };
 
 /* getVoxelIndex, src/fshader.glsl:33-52.  The shader's component tests
  * (y*W in [0, W*H), z*W*H in [0, W*H*W), x in [0, W)) are the per-axis
  * bounds; the index is formed in 64 bits (2048x512x2048 has 2^31 cells). */
-inline int64_t voxel_index(const Grid& g, int32_t x, int32_t y, int32_t z) {
+template <class W>
+inline int64_t voxel_index(const W& g, int32_t x, int32_t y, int32_t z) {
     if (x >= 0 && y >= 0 && z >= 0 && x < g.X && y < g.Y && z < g.Z)
         return (int64_t)x + g.X * (int64_t)y + g.X * g.Y * (int64_t)z;
     return -1;
@@ -125,7 +130,8 @@ struct ShaderGlobals {
 };
 
 /* castRay, src/fshader.glsl:59-129, transcribed statement by statement. */
-int64_t cast_ray(const Grid& g, const float startIn[3], const float rayDirection[3],
+template <class W>
+int64_t cast_ray(const W& g, const float startIn[3], const float rayDirection[3],
                  int32_t dist, int32_t renderDist, ShaderGlobals& G) {
     float startPosition[3] = {startIn[0], startIn[1], startIn[2]};
     int32_t currCheck[3] = {f2i(startPosition[0]), f2i(startPosition[1]), f2i(startPosition[2])};
@@ -178,12 +184,13 @@ int64_t cast_ray(const Grid& g, const float startIn[3], const float rayDirection
         }
         tempIndex = voxel_index(g, currCheck[0], currCheck[1], currCheck[2]);
 
-        if (tempIndex >= 0 && g.vox[tempIndex] >= 0) {
+        const int32_t word = tempIndex >= 0 ? g.at(tempIndex) : 0;
+        if (tempIndex >= 0 && word >= 0) {
             for (int a = 0; a < 3; a++) G.hitPos[a] = rayDirection[a] * currDist + startPosition[a];
             fColorIndex = tempIndex;
             break;
-        } else if (tempIndex >= 0 && g.vox[tempIndex] != -1) {
-            float toJump = -int_bits_to_float(g.vox[tempIndex]);
+        } else if (tempIndex >= 0 && word != -1) {
+            float toJump = -int_bits_to_float(word);
             distTravelled += toJump;
             if (!MUT(4)) currDist += toJump;  /* MUT(4): the jump does not move the start */
             if (MUT(13)) G.stepCount += 1.0f;  /* MUT(13): a jump counted as two steps */
@@ -227,7 +234,8 @@ void primary_dir(const vrt_frame& f, float vx, float vy, float out[3]) {
 }
 
 /* main(), src/fshader.glsl:131-190, for the fragment at vPos = (vx, vy). */
-void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut& o) {
+template <class W>
+void shade_pixel(const W& g, const vrt_frame& f, float vx, float vy, PixelOut& o) {
     ShaderGlobals G;
     float fColor[3] = {0.6f, 0.7f, 0.8f};
 
@@ -263,7 +271,7 @@ void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut
         normalize3(lv, toLight);
 
         float multiplier[3] = {0.4f, 0.4f, 0.4f};   /* AMBIENT */
-        if (fColorIndex != -1 && g.vox[fColorIndex] >= 0) {
+        if (fColorIndex != -1 && g.at(fColorIndex) >= 0) {
             float so[3];
             for (int a = 0; a < 3; a++) so[a] = firstHitPos[a] + toLight[a] * 0.001f;
             o.rays++;
@@ -309,7 +317,7 @@ void shade_pixel(const Grid& g, const vrt_frame& f, float vx, float vy, PixelOut
 
             if (MUT(24) && multiplier[0] >= 1.25f && multiplier[1] >= 1.25f && multiplier[2] >= 1.25f)
                 multiplier[0] = multiplier[1] = multiplier[2] = 1.25f;  /* MUT(24): clamp after the last light too */
-            int32_t v = g.vox[fColorIndex];
+            int32_t v = g.at(fColorIndex);
             if (MUT(19)) v = ((v & 0xFF) << 16) | (v & 0xFF00) | ((v >> 16) & 0xFF);  /* red and blue swapped */
             fColor[0] = (float)((v & 0x00FF0000) >> 16) / 255.0f * multiplier[0];
             fColor[1] = (float)((v & 0x0000FF00) >> 8) / 255.0f * multiplier[1];
@@ -359,10 +367,11 @@ const std::vector<DepthEntry>& depth_table() {
 
 /* fixDepthField, src/render.cpp:226-253; returns the new word of (x,y,z)
  * without writing, reading solidity from `g`. */
-int32_t fix_depth_value(const Grid& g, int32_t x, int32_t y, int32_t z) {
+template <class W>
+int32_t fix_depth_value(const W& g, int32_t x, int32_t y, int32_t z) {
     int64_t index = voxel_index(g, x, y, z);
     if (index < 0) return 0;
-    int32_t old = g.vox[index];
+    int32_t old = g.at(index);
     float dist = (float)(-VRT_DEPTH_FIELD_RADIUS + 1);
     float nearestDist = dist;
     if (old < 0) {
@@ -370,7 +379,7 @@ int32_t fix_depth_value(const Grid& g, int32_t x, int32_t y, int32_t z) {
         for (size_t i = 0; i < T.size(); i++) {
             dist = T[i].dist;
             int64_t indexCheck = voxel_index(g, x + T[i].x, y + T[i].y, z + T[i].z);
-            bool solid = indexCheck >= 0 && g.vox[indexCheck] >= 0;
+            bool solid = indexCheck >= 0 && g.at(indexCheck) >= 0;
             if (solid && dist > nearestDist) {
                 if (dist <= -2.0f || MUT(15)) nearestDist = dist;  /* MUT(15): -2 < d* stored too */
                 else nearestDist = 0.0f;
@@ -503,14 +512,18 @@ static inline void pixel_vpos(int32_t W, int32_t H, int64_t px, int64_t row, flo
     vy = (float)(2 * py + 1) / (float)H - 1.0f;
 }
 
+}  // extern "C"
+
+namespace {
+
 /* One frame.  pixels == NULL: every pixel, outputs indexed row*W + px
  * (row 0 = top), work split in 8x8 tiles with schedule(dynamic) like
  * fragment quads.  pixels != NULL: the listed pixel ids (row*W + px) only,
  * outputs indexed by list position.  totals[0] += rays, totals[1] += DDA
  * loop iterations. */
-void orc_render(const int32_t* vox, int64_t X, int64_t Y, int64_t Z, const vrt_frame* f, int32_t W, int32_t H,
-                const int64_t* pixels, int64_t npix, const orc_outputs* o, uint64_t* totals, int32_t threads) {
-    Grid g{vox, X, Y, Z};
+template <class Wd>
+void render_frame(const Wd& g, const vrt_frame* f, int32_t W, int32_t H, const int64_t* pixels, int64_t npix,
+                  const orc_outputs* o, uint64_t* totals, int32_t threads) {
     uint64_t rays = 0, iters = 0;
     if (pixels == nullptr) {
         int64_t tx = (W + 7) / 8, ty = (H + 7) / 8;
@@ -529,7 +542,7 @@ void orc_render(const int32_t* vox, int64_t X, int64_t Y, int64_t Z, const vrt_f
                 }
         }
     } else {
-#pragma omp parallel for schedule(dynamic, 64) num_threads(threads) reduction(+ : rays, iters)
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) reduction(+ : rays, iters)
         for (int64_t k = 0; k < npix; k++) {
             int64_t id = pixels[k];
             float vx, vy;
@@ -545,6 +558,127 @@ void orc_render(const int32_t* vox, int64_t X, int64_t Y, int64_t Z, const vrt_f
         totals[0] += rays;
         totals[1] += iters;
     }
+}
+
+/* The world a render sees after removeSphere edits, evaluated on demand
+ * from the raw world (solid colours and -1, no depth field) and the edit
+ * log, for worlds too large to build the depth field of on the CPU.  It
+ * stands for: computeDepthField over the raw world (src/render.cpp:273-286),
+ * then removeSphere (src/level.cpp:30-56) for each of the first T edits.
+ *
+ * At version k (k edits applied) a cell is solid iff it is solid in the raw
+ * world and no edit j <= k carved it (offset o from the centre with
+ * |o|^2 < r_j^2).  fixDepthField reads only solidity and the cell's old
+ * word, and the old word of an empty cell it keeps is always -1: removals
+ * only lower the nearest-solid distance d*, so a cell with d* > -2 now had
+ * d* > -2 at every earlier fix since it became empty (a raw empty cell, or
+ * a carved one, which the carve set to -1).  So the word of an empty cell at
+ * version T is fixDepthField at version k, with k the last edit <= T whose
+ * repair ball (|o|^2 < (r_k + 3)^2) holds the cell (a carved cell lies in
+ * its own edit's ball), or k = 0, the initial build.  The same
+ * fix_depth_value as the eager build computes it, on a view that answers
+ * solid colour or -1 at version k; version-0 words are memoised. */
+struct EditRec {
+    int32_t cx, cy, cz, r;
+};
+
+struct History {
+    const int32_t* raw;
+    int64_t X, Y, Z;
+    const EditRec* edits;
+    int32_t n;
+
+    void xyz(int64_t i, int32_t& x, int32_t& y, int32_t& z) const {
+        x = (int32_t)(i % X);
+        y = (int32_t)((i / X) % Y);
+        z = (int32_t)(i / (X * Y));
+    }
+    static int64_t d2(const EditRec& e, int32_t x, int32_t y, int32_t z) {
+        const int64_t dx = (int64_t)x - e.cx, dy = (int64_t)y - e.cy, dz = (int64_t)z - e.cz;
+        return dx * dx + dy * dy + dz * dz;
+    }
+    /* carved by one of the first k edits (removeSphere's first loop) */
+    bool carved(int32_t x, int32_t y, int32_t z, int32_t k) const {
+        for (int32_t j = 0; j < k; j++)
+            if (edits[j].r > 0 && d2(edits[j], x, y, z) < (int64_t)edits[j].r * edits[j].r) return true;
+        return false;
+    }
+    /* last edit <= k whose repair loop (radius r + 3) visits the cell, 1-based; 0 = none */
+    int32_t last_repair(int32_t x, int32_t y, int32_t z, int32_t k) const {
+        for (int32_t j = k - 1; j >= 0; j--) {
+            const int64_t R = (int64_t)edits[j].r + (VRT_DEPTH_FIELD_RADIUS >> 1) - (MUT(25) ? 1 : 0);  /* MUT(25): ball of r + 2 */
+            if (R > 0 && d2(edits[j], x, y, z) < R * R) return j + 1;
+        }
+        return 0;
+    }
+};
+
+/* Words as fixDepthField sees them at version k: the solid colour, or -1. */
+struct SolidAt {
+    const History* h;
+    int32_t k;
+    int64_t X, Y, Z;
+    int32_t at(int64_t i) const {
+        const int32_t v = h->raw[i];
+        if (v < 0) return -1;
+        int32_t x, y, z;
+        h->xyz(i, x, y, z);
+        return h->carved(x, y, z, k) ? -1 : v;
+    }
+};
+
+/* Words a render sees at version T. */
+struct LazyWorld {
+    const History* h;
+    int32_t T;
+    int32_t* memo;  /* version-0 word of each raw-empty cell once computed; 0 = not yet (never a result) */
+    int64_t X, Y, Z;
+    int32_t at(int64_t i) const {
+        const int32_t v = h->raw[i];
+        int32_t x, y, z;
+        h->xyz(i, x, y, z);
+        if (v >= 0 && !h->carved(x, y, z, T)) return v;
+        const int32_t k = MUT(26) && v < 0 ? 0 : h->last_repair(x, y, z, T);  /* MUT(26): raw-empty cells keep the first build's word */
+        if (k == 0) {
+            int32_t m = __atomic_load_n(memo + i, __ATOMIC_RELAXED);
+            if (m != 0) return m;
+            m = fix_depth_value(SolidAt{h, 0, X, Y, Z}, x, y, z);
+            __atomic_store_n(memo + i, m, __ATOMIC_RELAXED);
+            return m;
+        }
+        return fix_depth_value(SolidAt{h, k, X, Y, Z}, x, y, z);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+void orc_render(const int32_t* vox, int64_t X, int64_t Y, int64_t Z, const vrt_frame* f, int32_t W, int32_t H,
+                const int64_t* pixels, int64_t npix, const orc_outputs* o, uint64_t* totals, int32_t threads) {
+    Grid g{vox, X, Y, Z};
+    render_frame(g, f, W, H, pixels, npix, o, totals, threads);
+}
+
+/* orc_render on the world after the first T edits of `edits` ((cx, cy, cz,
+ * r) each) applied to `raw` with its depth field built first, evaluated on
+ * demand.  raw's empty cells must be -1 (no depth field).  memo: X*Y*Z
+ * int32 zeros, kept across calls on the same raw world. */
+void orc_render_lazy(const int32_t* raw, int64_t X, int64_t Y, int64_t Z, const int32_t* edits, int32_t T,
+                     int32_t* memo, const vrt_frame* f, int32_t W, int32_t H, const int64_t* pixels, int64_t npix,
+                     const orc_outputs* o, uint64_t* totals, int32_t threads) {
+    History h{raw, X, Y, Z, reinterpret_cast<const EditRec*>(edits), T};
+    LazyWorld g{&h, T, memo, X, Y, Z};
+    render_frame(g, f, W, H, pixels, npix, o, totals, threads);
+}
+
+/* The words of the listed cells (linear indices) at version T, as above. */
+void orc_lazy_words(const int32_t* raw, int64_t X, int64_t Y, int64_t Z, const int32_t* edits, int32_t T,
+                    int32_t* memo, const int64_t* idx, int64_t n, int32_t* out, int32_t threads) {
+    History h{raw, X, Y, Z, reinterpret_cast<const EditRec*>(edits), T};
+    LazyWorld g{&h, T, memo, X, Y, Z};
+#pragma omp parallel for schedule(dynamic, 256) num_threads(threads)
+    for (int64_t k = 0; k < n; k++) out[k] = g.at(idx[k]);
 }
 
 int orc_max_threads(void) { return omp_get_max_threads(); }

@@ -38,6 +38,9 @@ def lib():
         L.orc_remove_sphere.argtypes = [P, i64, i64, i64, i32, i32, i32, i32]
         L.orc_cast_rays.argtypes = [P, i64, i64, i64, i64, P, P, P, i32, P, P, P, P]
         L.orc_render.argtypes = [P, i64, i64, i64, P, i32, i32, P, i64, C.POINTER(orc_outputs), P, i32]
+        L.orc_render_lazy.argtypes = [P, i64, i64, i64, P, i32, P, P, i32, i32, P, i64, C.POINTER(orc_outputs), P,
+                                      i32]
+        L.orc_lazy_words.argtypes = [P, i64, i64, i64, P, i32, P, P, i64, P, i32]
         L.orc_max_threads.restype = C.c_int
         L.orc_normalize.argtypes = [P, P]
         L.orc_length.argtypes = [P]
@@ -127,6 +130,57 @@ def render(world_df, dims, frame, width, height, pixels=None, nthreads=None):
     out["rays"] = int(totals[0])
     out["dda_steps"] = int(totals[1])
     return out
+
+
+class LazyWorld:
+    """The world after its depth-field build and the first `version` edits of
+    an edit log, evaluated by the oracle on demand from the raw world (solid
+    colours and -1): for worlds too large for orc.depth_field on the CPU.
+    edits: (cx, cy, cz, r) removeSphere calls in order; the memo of the
+    version-0 words is kept across calls."""
+
+    def __init__(self, raw, dims, edits=()):
+        self.raw = np.ascontiguousarray(raw, np.int32)
+        self.dims = tuple(int(d) for d in dims)
+        assert self.raw.size == self.dims[0] * self.dims[1] * self.dims[2]
+        self.memo = np.zeros(self.raw.size, np.int32)
+        self.set_edits(edits)
+
+    def set_edits(self, edits):
+        self.edits = np.ascontiguousarray(np.asarray(list(edits), np.int32).reshape(-1, 4))
+
+    def render(self, frame, width, height, version=None, pixels=None, nthreads=None):
+        T = len(self.edits) if version is None else int(version)
+        assert 0 <= T <= len(self.edits)
+        if pixels is None:
+            shape, n, pix = (height, width), width * height, None
+        else:
+            pix = np.ascontiguousarray(pixels, np.int64)
+            shape, n = (pix.shape[0],), pix.shape[0]
+        out = {"rgb": np.empty(shape + (3,), np.uint8), "depth": np.empty(shape, np.float32),
+               "hit": np.empty(shape, np.int64), "face": np.empty(shape, np.uint8),
+               "shadow_eval": np.empty(shape, np.uint32), "shadow_occ": np.empty(shape, np.uint32),
+               "steps": np.empty(shape, np.uint32)}
+        o = orc_outputs(_p(out["rgb"]), _p(out["depth"]), _p(out["hit"]), _p(out["face"]), _p(out["shadow_eval"]),
+                        _p(out["shadow_occ"]), _p(out["steps"]))
+        totals = np.zeros(2, np.uint64)
+        X, Y, Z = self.dims
+        lib().orc_render_lazy(_p(self.raw), X, Y, Z, _p(self.edits), T, _p(self.memo), C.byref(frame), int(width),
+                              int(height), _p(pix), n if pix is not None else 0, C.byref(o), _p(totals),
+                              nthreads or threads())
+        out["rays"] = int(totals[0])
+        out["dda_steps"] = int(totals[1])
+        return out
+
+    def words(self, idx, version=None, nthreads=None):
+        """The words of the listed linear cell indices at `version`."""
+        T = len(self.edits) if version is None else int(version)
+        idx = np.ascontiguousarray(idx, np.int64).reshape(-1)
+        out = np.empty(idx.size, np.int32)
+        X, Y, Z = self.dims
+        lib().orc_lazy_words(_p(self.raw), X, Y, Z, _p(self.edits), T, _p(self.memo), _p(idx), idx.size, _p(out),
+                             nthreads or threads())
+        return out
 
 
 def world_reference(dims=(512, 96, 512)):

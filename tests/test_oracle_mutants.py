@@ -5,8 +5,8 @@ makes one deliberate mistake of the kind a transcription of fshader.glsl,
 render.cpp or level.cpp could make (a dropped term, a flipped sign or
 comparison, a transposed operand, a wrong rounding).  Each mutant is built
 into a temporary directory and tests/test_oracle_pins.py is run against it in
-a child process (VRT_ORACLE_LIB points the loader at the mutant); the pins
-must fail on every one of them.  A mutant that survives would be a mistake
+a child process together with tests/test_oracle_lazy.py (VRT_ORACLE_LIB
+points the loader at the mutant); the pins must fail on every one of them.  A mutant that survives would be a mistake
 the oracle could carry unnoticed, and through it the kernels."""
 import os
 import subprocess
@@ -44,6 +44,8 @@ MUTANTS = {
     22: "depth view scaled by 1/255 instead of 1/100",
     23: "aspect ratio applied to y",
     24: "overbright clamp applied after the last light too",
+    25: "on-demand world: repair ball of radius r + 2",
+    26: "on-demand world: raw-empty cells keep the first build's word after edits",
 }
 
 
@@ -68,8 +70,8 @@ def mutant_libs(tmp_path_factory):
 
 def run_pins(lib):
     env = dict(os.environ, VRT_ORACLE_LIB=lib, OMP_NUM_THREADS="2")
-    cmd = [sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle_pins.py"), "-x", "-q",
-           "-p", "no:cacheprovider"]
+    cmd = [sys.executable, "-m", "pytest", os.path.join(ROOT, "tests", "test_oracle_pins.py"),
+           os.path.join(ROOT, "tests", "test_oracle_lazy.py"), "-x", "-q", "-p", "no:cacheprovider"]
     return subprocess.run(cmd, env=env, cwd=ROOT, capture_output=True, text=True, timeout=600)
 
 
