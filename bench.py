@@ -47,19 +47,49 @@ def load():
     return vrt
 
 
+# the sources k_render_plain is compiled from: a counter summary applies to
+# the kernel only while these are unchanged
+KERNEL_SOURCES = ["voxel-rt_amd/csrc/vrt_device.h", "voxel-rt_amd/csrc/vrt_render.hip",
+                  "voxel-rt_amd/csrc/vrt_internal.h"]
+COUNTERS = os.path.join(ROOT, "profiles", "k_render_plain_1080p_16lights_counters.md")
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update(open(os.path.join(ROOT, f), "rb").read())
+    return h.hexdigest()[:16]
+
+
 def profiled(key):
     """A per-launch figure of k_render_plain measured with rocprofv3 --pmc, from
     the committed counter summary: "traffic_bytes_per_launch" (calibrated
     FETCH_SIZE + WRITE_SIZE), "valu_lane_ops_per_launch"
     (SQ_THREAD_CYCLES_VALU, one per active lane per VALU instruction) or
     "valu_issue_frac" (4 x SQ_INSTS_VALU over 1024 SIMDs x the launch's
-    cycles: the share of VALU issue slots used)."""
-    path = os.path.join(ROOT, "profiles", "k_render_plain_1080p_16lights_counters.md")
+    cycles: the share of VALU issue slots used).  None when the summary was
+    taken on other kernel sources (its kernel_source_sha16 line differs from
+    the current sources' hash): a stale figure is not reported."""
+    vals = {}
     try:
-        for line in open(path):
-            if line.startswith(key + ":"):
-                v = line.split(":")[1]
-                return float(v) if "." in v else int(v)
+        for line in open(COUNTERS):
+            if ":" in line and not line.startswith(("|", " ", "#")):
+                k, v = line.split(":", 1)
+                vals[k.strip()] = v.strip()
+    except OSError:
+        return None
+    if vals.get("kernel_source_sha16") != kernel_source_hash() or key not in vals:
+        return None
+    v = vals[key]
+    return float(v) if "." in v else int(v)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
     except OSError:
         pass
     return None
@@ -123,7 +153,7 @@ def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
         valu_achieved = lane_ops / (kernel_ms * 1e-3) / 1e12
         valu = {"achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "T lane-ops/s",
                 "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops,
-                "issue_frac_profiled": profiled("valu_issue_frac")}
+                "issue_frac_profiled": profiled("valu_issue_frac"), "limiter": True}
     # The same frames with the cutoff off: every output is identical,
     # the DDA walks are the reference's full length (its iteration count).
     full = None
@@ -135,7 +165,10 @@ def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
         "rays_per_frame": int(rays_per_frame), "dda_steps_per_frame": int(steps_per_frame),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": profiled("traffic_bytes_per_launch"),
-                     "bytes_per_launch": int(bytes_per_launch), "kernel": "k_render_plain", "valu": valu},
+                     "bytes_per_launch": int(bytes_per_launch), "kernel": "k_render_plain",
+                     "note": "achieved = algorithmic bytes (4 B per DDA iteration + 7 B per pixel) over the "
+                             "kernel time; the measured HBM traffic is below them (the grid is served from L2) "
+                             "and the kernel is bound by VALU issue (valu)", "valu": valu},
         "shadow_cutoff_off": full,
     }
     return sc, ctx, out
@@ -234,7 +267,39 @@ def batched_bench(vrt, reps=5):
     st2 = ctx.stats(reset=True)
     assert st2["rays"] // reps == rays, "batched and plain cast different ray counts"
     vrt.call("vrt_batch_destroy", b)
+    # Per resolution class: the class's plain launches (from the runs above)
+    # against one tuned batch of that class alone.  A batch of all 64 views
+    # cannot take less time than its largest class alone, so plain_ms over
+    # the 4K class's batched time bounds the speedup any scheduling can reach.
+    per_view = np.median(np.array([np.diff([0.0] + d) for d in plain_done]), axis=0)
+    classes = {}
+    for i, (_, w, h) in enumerate(views):
+        classes.setdefault((w, h), []).append(i)
+    by_class = []
+    for (w, h), idx in sorted(classes.items(), key=lambda kv: kv[0][0] * kv[0][1]):
+        sub = (vrt.vrt_view * len(idx))()
+        for j, i in enumerate(idx):
+            sub[j] = arr[i]
+        bc = C.c_void_p()
+        vrt.call("vrt_batch_create", ctx.h, sub, len(idx), C.byref(best), C.byref(bc))
+        vrt.call("vrt_batch_launch", bc)
+        ctx.sync()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = vrt.Event(), vrt.Event()
+            e0.record(stream)
+            vrt.call("vrt_batch_launch", bc)
+            e1.record(stream)
+            ctx.sync()
+            ms.append(e1.ms_since(e0))
+        vrt.call("vrt_batch_destroy", bc)
+        pms = float(np.sum(per_view[idx]))
+        by_class.append({"resolution": [w, h], "views": len(idx), "pixels": int(len(idx) * w * h),
+                         "plain_ms": pms, "batched_alone_ms": float(np.median(ms)),
+                         "speedup": pms / float(np.median(ms))})
+    ctx.stats(reset=True)
     ctx.close()
+    largest = by_class[-1]
     pm, bm = float(np.median(plain_ms)), float(np.median(batch_ms))
     done = np.median(np.array(plain_done), axis=0)
     # algorithmic bytes of the set: 4 B per DDA iteration, RGB8 + f32 depth per pixel
@@ -250,36 +315,59 @@ def batched_bench(vrt, reps=5):
                      "plain_achieved": alg / (pm * 1e-3) / 1e9, "plain_frac": alg / (pm * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "batched_achieved": alg / (bm * 1e-3) / 1e9,
                      "batched_frac": alg / (bm * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "by_class": by_class,
+        "speedup_bound": {"value": pm / largest["batched_alone_ms"],
+                          "note": "64 plain launches over one batch of the %dx%d views alone (%d of %d pixels): "
+                                  "the whole set batched cannot be faster than that class batched"
+                                  % (largest["resolution"][0], largest["resolution"][1], largest["pixels"], pixels)},
         "tuned": {"tile": [best.tile_w, best.tile_h], "chunk_tiles": best.chunk_tiles, "tune_s": tune_s,
                   "sweep": [{"tile": [res[i].config.tile_w, res[i].config.tile_h],
                              "chunk": res[i].config.chunk_tiles, "ms": res[i].ms} for i in range(n.value)]},
     }
 
 
-def served_bench(vrt, n_requests=240, load=0.7):
-    """Resident 1024x512x1024 world; Poisson render requests (resolutions as
-    in the batched set) at `load` x the plain throughput, 10 removeSphere edits
-    (radius 1-8 at surface points) per 100 requests, 32 GB framebuffer budget."""
+def views64_profile(vrt, launches=3):
+    """For counter runs: the 64-view set drawn once as 64 plain launches (the
+    first 64 k_render_plain dispatches), the autotuner's quick pass, then
+    `launches` launches of the tuned batch (the last batch-kernel dispatches)."""
     C = vrt.C
-    params, dims, world, ctx = world_ctx(vrt, "served2g", 5)
-    X, Y, Z = dims
-    views = vrt.make_views(params, dims, world, 55, n_requests)
+    params, dims, world, ctx = world_ctx(vrt, "views64", 4)
+    views = vrt.make_views(params, dims, world, 44, 64)
     del world
-    rng = np.random.default_rng(56)
-    # offered rate from a plain pass over a sample of the requests
-    probe = views[:16]
-    tg = [vrt.Target(w, h, depth=True) for (_, w, h) in probe]
+    targets = [vrt.Target(w, h, depth=True) for (_, w, h) in views]
+    arr = (vrt.vrt_view * len(views))()
+    for i, ((f, w, h), t) in enumerate(zip(views, targets)):
+        arr[i].frame = f
+        arr[i].target = t.struct()
+    vrt.call("vrt_draw_views_plain", ctx.h, arr, len(views))
     ctx.sync()
-    e0, e1 = vrt.Event(), vrt.Event()
-    e0.record(ctx.stream)
-    for (f, w, h), t in zip(probe, tg):
-        ctx.draw(t, frame=f)
-    e1.record(ctx.stream)
+    best = vrt.vrt_batch_config()
+    vrt.call("vrt_autotune_batch", ctx.h, arr, len(views), 1, C.byref(best), None, 0, None)
+    b = C.c_void_p()
+    vrt.call("vrt_batch_create", ctx.h, arr, len(views), C.byref(best), C.byref(b))
     ctx.sync()
-    mean_ms = e1.ms_since(e0) / len(probe)
-    del tg
-    rate = load / (mean_ms * 1e-3)  # requests per second
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    ctx.stats(reset=True)
+    for _ in range(launches):
+        vrt.call("vrt_batch_launch", b)
+    ctx.sync()
+    st = ctx.stats(reset=True)
+    vrt.call("vrt_batch_destroy", b)
+    ctx.close()
+    return {"tuned": {"tile": [best.tile_w, best.tile_h], "chunk_tiles": best.chunk_tiles},
+            "rays_per_launch": st["rays"] // launches, "dda_steps_per_launch": st["dda_steps"] // launches,
+            "pixels": int(sum(w * h for (_, w, h) in views))}
+
+
+def served_stream(vrt, ctx, params, dims, views, rate, edit_seed):
+    """One request stream through a fresh server: requests arrive at Poisson
+    times of `rate` per second (rate None: all at once, the saturation run),
+    a removeSphere edit (radius 1-8 at a surface point) before every tenth.
+    Returns per-request times on the server's clock and the stream's totals."""
+    C = vrt.C
+    X, Y, Z = dims
+    n = len(views)
+    rng = np.random.default_rng(edit_seed)
+    arrivals = np.zeros(n) if rate is None else np.cumsum(rng.exponential(1.0 / rate, n))
     cfg = vrt.vrt_server_config()
     cfg.framebuffer_budget = 32 << 30
     cfg.max_batch_views = 16
@@ -290,32 +378,93 @@ def served_bench(vrt, n_requests=240, load=0.7):
     vrt.call("vrt_server_create", ctx.h, C.byref(cfg), C.byref(s))
     ctx.stats(reset=True)
     res = (vrt.vrt_result * 64)()
-    n = C.c_int32(0)
-    lat, edits, nxt, done = [], 0, 0, 0
+    got = C.c_int32(0)
+    rows, edits, nxt = [], 0, 0
+    after_edit = set()
     t_start = vrt.lib().vrt_server_clock(s)
-    while done < n_requests:
+    while len(rows) < n:
         now = vrt.lib().vrt_server_clock(s) - t_start
-        while nxt < n_requests and arrivals[nxt] <= now:
+        while nxt < n and arrivals[nxt] <= now:
             if nxt % 10 == 9:  # 10 edits per 100 requests
                 x, z = int(rng.integers(0, X)), int(rng.integers(0, Z))
-                y = vrt.surface(params, x, z)
-                vrt.call("vrt_server_submit_edit", s, x, y, z, int(rng.integers(1, 9)), C.byref(C.c_uint64()))
+                vrt.call("vrt_server_submit_edit", s, x, vrt.surface(params, x, z), z, int(rng.integers(1, 9)),
+                         C.byref(C.c_uint64()))
                 edits += 1
             f, w, h = views[nxt]
-            vrt.call("vrt_server_submit", s, C.byref(f), w, h, float(t_start + arrivals[nxt]), C.byref(C.c_uint64()))
+            rid = C.c_uint64()
+            vrt.call("vrt_server_submit", s, C.byref(f), w, h, float(t_start + arrivals[nxt]), C.byref(rid))
+            if nxt % 10 == 9:
+                after_edit.add(rid.value)
             nxt += 1
-        vrt.call("vrt_server_pump", s, 0)
-        vrt.call("vrt_server_poll", s, res, 64, C.byref(n))
-        for i in range(n.value):
-            lat.append((res[i].t_done - res[i].t_submit) * 1e3)
-            vrt.call("vrt_server_release", s, res[i].id)
-        done += n.value
+        vrt.call("vrt_server_pump", s, 1 if nxt == n else 0)
+        vrt.call("vrt_server_poll", s, res, 64, C.byref(got))
+        for i in range(got.value):
+            r = res[i]
+            rows.append((r.t_submit, r.t_start, r.t_exec, r.t_gpu_done, r.t_done, r.width * r.height,
+                         r.id in after_edit))
+            vrt.call("vrt_server_release", s, r.id)
     wall = vrt.lib().vrt_server_clock(s) - t_start
     st = ctx.stats(reset=True)
-    alg = 4 * st["dda_steps"] + 7 * sum(w * h for (_, w, h) in views)
     vrt.call("vrt_server_destroy", s)
-    # one removeSphere (carve + depth-field repair) on its own, device time
-    # between events on the context's stream, radius 1-8 at surface points
+    t = np.array([r[:5] for r in rows])
+    return {"t": t, "pixels": np.array([r[5] for r in rows]), "after_edit": np.array([r[6] for r in rows]),
+            "wall": wall, "rays": st["rays"], "dda_steps": st["dda_steps"], "edits": edits}
+
+
+def served_summary(run, label):
+    """Latency percentiles and, for the slowest 1% of requests, where their
+    time went: queue (submitted -> launched by the host), device wait
+    (launched -> its batch began: an edit's render-copy barrier and the
+    stream's earlier work), render (on the device) and notice (device done ->
+    the host saw it)."""
+    t = run["t"]
+    lat = (t[:, 4] - t[:, 0]) * 1e3
+    parts = {"queue": t[:, 1] - t[:, 0], "device_wait": t[:, 2] - t[:, 1], "render": t[:, 3] - t[:, 2],
+             "notice": t[:, 4] - t[:, 3]}
+    slow = lat >= np.percentile(lat, 99)
+    busy = t[:, 3].max() - t[:, 2].min()
+    return {
+        "load": label, "requests": int(len(lat)), "edits": int(run["edits"]),
+        "latency_ms": percentiles(lat),
+        "parts_ms": {k: {"p50": float(np.percentile(v, 50) * 1e3), "p99": float(np.percentile(v, 99) * 1e3)}
+                     for k, v in parts.items()},
+        "slowest_1pct_mean_ms": {k: float(v[slow].mean() * 1e3) for k, v in parts.items()},
+        "slowest_1pct_after_edit": int(run["after_edit"][slow].sum()),
+        "mrays_s": run["rays"] / run["wall"] / 1e6, "wall_s": run["wall"],
+        "device_span_s": float(busy), "requests_per_s": len(lat) / run["wall"],
+    }
+
+
+def served_bench(vrt, n_requests=240, loads=(0.5, 0.9, 1.2, 2.0)):
+    """Resident 1024x512x1024 world; render requests with the batched set's
+    resolution mix, 10 removeSphere edits (radius 1-8 at surface points) per
+    100 requests, 32 GB framebuffer budget.  First all requests at once (the
+    server at saturation: its capacity in requests/s and Mrays/s), then
+    Poisson streams offered at fractions and multiples of that capacity."""
+    params, dims, world, ctx = world_ctx(vrt, "served2g", 5)
+    views = vrt.make_views(params, dims, world, 55, n_requests)
+    del world
+    sat = served_stream(vrt, ctx, params, dims, views, None, 56)
+    cap_rps = len(views) / sat["wall"]
+    out = {"workload": "resident 1024x512x1024 grid (2 GiB), %d requests per stream, resolutions as the batched "
+                       "set, 10 removeSphere edits (r 1-8) per 100 requests, 32 GB framebuffer budget, 3 slots of up "
+                       "to 16 views" % n_requests,
+           "saturation": served_summary(sat, "all at once")}
+    out["capacity"] = {"requests_per_s": cap_rps, "mrays_s": sat["rays"] / sat["wall"] / 1e6,
+                       "mrays_s_device_span": sat["rays"] / out["saturation"]["device_span_s"] / 1e6}
+    alg = 4 * sat["dda_steps"] + 7 * int(sat["pixels"].sum())
+    out["roofline"] = {"bound": "hbm", "bytes": int(alg), "achieved": alg / sat["wall"] / 1e9, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": alg / sat["wall"] / 1e9 / HBM_PEAK_GBS,
+                       "note": "algorithmic bytes of the saturation run over its wall clock"}
+    out["loads"] = []
+    for k, load in enumerate(loads):
+        run = served_stream(vrt, ctx, params, dims, views, load * cap_rps, 57 + k)
+        out["loads"].append(served_summary(run, "%.1fx capacity (%.0f req/s)" % (load, load * cap_rps)))
+    # one removeSphere (carve + depth-field repair + render-copy box) on its
+    # own, device time between events on the context's stream
+    rng = np.random.default_rng(58)
+    X, Y, Z = dims
+    e0, e1 = vrt.Event(), vrt.Event()
     edit_us = []
     for _ in range(40):
         x, z = int(rng.integers(8, X - 8)), int(rng.integers(8, Z - 8))
@@ -326,19 +475,9 @@ def served_bench(vrt, n_requests=240, load=0.7):
         ctx.sync()
         edit_us.append(e1.ms_since(e0) * 1e3)
     ctx.close()
-    return {
-        "workload": "resident 1024x512x1024 grid (2 GiB), %d Poisson requests at %.0f req/s (%.0f%% of the plain "
-                    "rate), resolutions as the batched set, %d removeSphere edits (r 1-8), 32 GB framebuffer budget"
-                    % (n_requests, rate, 100 * load, edits),
-        "latency_ms": percentiles(lat), "mrays_s": st["rays"] / wall / 1e6, "rays": int(st["rays"]),
-        "wall_s": wall, "plain_ms_per_request": mean_ms,
-        "roofline": {"bound": "hbm", "bytes": int(alg), "achieved": alg / wall / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": alg / wall / 1e9 / HBM_PEAK_GBS,
-                     "note": "over the wall clock of the request stream, offered at %.0f%% of the plain rate"
-                             % (100 * load)},
-        "remove_sphere_us": {"p50": float(np.percentile(edit_us, 50)), "p99": float(np.percentile(edit_us, 99)),
-                             "sample": "40 edits, radius 1-8 at surface points, carve + repair"},
-    }
+    out["remove_sphere_us"] = {"p50": float(np.percentile(edit_us, 50)), "p99": float(np.percentile(edit_us, 99)),
+                               "sample": "40 edits, radius 1-8 at surface points, carve + repair + copy"}
+    return out
 
 
 def large_bench(vrt, frames=10):
@@ -399,7 +538,7 @@ def cpu_baseline(vrt, sc, ctx):
         times.append(time.perf_counter() - t)
         rays = r["rays"]
     med = float(np.median(times))
-    return {"value": rays / med / 1e6, "unit": "Mrays/s", "cores": n, "kind": "port",
+    return {"value": rays / med / 1e6, "unit": "Mrays/s", "cores": n, "kind": "port", "cpu_model": cpu_model(),
             "sample": "full 1920x1080 frames of the same scene for ~10 s after 1 warm-up: median of %d "
                       "(%.3f s/frame)" % (len(times), med)}
 
@@ -411,7 +550,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
-    ap.add_argument("--only", choices=["plain", "batched", "served", "large"], default=None,
+    ap.add_argument("--only", choices=["plain", "batched", "served", "large", "views64_profile"], default=None,
                     help="run only this leg (for profiling); prints its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -419,6 +558,9 @@ def main():
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     vrt = load()
+    if args.only == "views64_profile":
+        print(json.dumps(views64_profile(vrt)), flush=True)
+        return
     if args.only == "batched":
         print(json.dumps(batched_bench(vrt, reps=2)), flush=True)
         return
@@ -458,9 +600,11 @@ def main():
             res["batched"]["plain_ms"], res["batched"]["batched_ms"], res["batched"]["speedup"],
             res["batched"]["roofline"]["plain_frac"], res["batched"]["roofline"]["batched_frac"]))
         res["served"] = served_bench(vrt)
-        log("served: p50 %.2f ms p99 %.2f ms, %.0f Mrays/s, HBM roofline %.2f over the wall clock" % (
-            res["served"]["latency_ms"]["p50"], res["served"]["latency_ms"]["p99"], res["served"]["mrays_s"],
-            res["served"]["roofline"]["frac"]))
+        sv = res["served"]
+        log("served: capacity %.0f req/s, %.0f Mrays/s at saturation; " % (
+            sv["capacity"]["requests_per_s"], sv["capacity"]["mrays_s"]) + ", ".join(
+            "%s p50 %.2f p99 %.2f ms" % (L["load"], L["latency_ms"]["p50"], L["latency_ms"]["p99"])
+            for L in sv["loads"]))
         res["large"] = large_bench(vrt)
         log("large: shaded %.2f ms (%.0f Mrays/s), depth view %.2f ms, depth field %.0f ms" % (
             res["large"]["shaded"]["kernel_ms"], res["large"]["shaded"]["mrays_s"],

@@ -150,13 +150,26 @@ int vrt_upload(vrt_ctx* ctx, const int32_t* host_voxels);
 /* Full grid device->device (grid built or kept elsewhere on the GPU). */
 int vrt_upload_device(vrt_ctx* ctx, const int32_t* device_voxels);
 /* Row-wise partial upload of the box [x0,x1] x [y0,y1) x [z0,z1) of a full
- * host grid image; updatePartialGeometry, src/render.cpp:204-223 (which
- * uploads x-rows of length x1-x0+1 for y in [y0,y1), z in [z0,z1), after
- * swapping the corners when start is above end).  Coordinates are clipped
- * to the grid. */
+ * host grid image; updatePartialGeometry, src/render.cpp:204-223, for
+ * integral corners (which uploads x-rows of length x1-x0+1 for y in [y0,y1),
+ * z in [z0,z1), after swapping the corners when getVoxelIndex of start is
+ * above end's).  As there, nothing is uploaded when x0 lies outside the grid
+ * (every row's getVoxelIndex is -1) and rows outside the grid are skipped;
+ * unlike there, a row is clipped at the grid's last x instead of running on
+ * into the next row, so device-side edits there are never overwritten. */
 int vrt_upload_region(vrt_ctx* ctx, const int32_t* host_voxels,
                       int32_t x0, int32_t y0, int32_t z0,
                       int32_t x1, int32_t y1, int32_t z1);
+/* updatePartialGeometry as the reference writes it, with float corners
+ * (src/render.cpp:204-223): the rows its float loops visit (for float i from
+ * start.z while i < end.z, i++; likewise j over y), each a run of
+ * int(end.x - start.x) + 1 words from getVoxelIndex(int(start.x), int(j),
+ * int(i)); runs that start outside the grid are skipped, a run that passes
+ * the row's end continues into the next row as glBufferSubData does, and a
+ * run past the grid's end is not copied.  VRT_ERR_INVALID for corners that
+ * are not finite or reach 2^24 in magnitude (the reference's loops would not
+ * end). */
+int vrt_upload_region_f(vrt_ctx* ctx, const int32_t* host_voxels, const float start[3], const float end[3]);
 /* Full grid device->host (the reference's host array is always current;
  * this returns the device's copy after device-side edits). */
 int vrt_download(vrt_ctx* ctx, int32_t* host_voxels);
@@ -269,7 +282,10 @@ int vrt_autotune_batch(vrt_ctx* ctx, const vrt_view* views, int32_t n, int32_t r
 /* ---- served form: resident world, edits between frames, continuous batching */
 
 typedef struct vrt_server_config {
-    uint64_t framebuffer_budget;   /* bytes of frame buffers admitted at once */
+    uint64_t framebuffer_budget;   /* bytes of frame buffers admitted at once; the server reserves
+                                    * this much device memory at creation and sub-allocates every
+                                    * request's buffers from it (no device allocation or free while
+                                    * serving) */
     int32_t max_batch_views;       /* views per batch launch */
     int32_t streams;               /* batches that may run on the device together */
     int32_t want_depth;            /* allocate and fill depth per request */
@@ -281,7 +297,13 @@ typedef struct vrt_result {
     uint64_t id;
     uint64_t world_version;        /* edits applied to the world this frame saw */
     int32_t width, height;
-    double t_submit, t_start, t_done;  /* seconds on the server's clock */
+    /* Seconds on the server's clock (vrt_server_clock).  t_submit: as given to
+     * vrt_server_submit; t_start: admitted and launched by the host; t_exec:
+     * its batch began on the device (after any edit it had to see and earlier
+     * work on its stream); t_gpu_done: its batch ended on the device; t_done:
+     * the host saw it complete.  The two device times come from HIP events,
+     * placed on the host clock through one event synchronised at creation. */
+    double t_submit, t_start, t_exec, t_gpu_done, t_done;
     vrt_target target;             /* device buffers, valid until vrt_server_release */
 } vrt_result;
 

@@ -302,6 +302,11 @@ int solid_top_full(vrt_ctx* c) {
 }
 
 int refresh_render(vrt_ctx* c, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1, int64_t z1) {
+    return refresh_render_on(c, c->stream, x0, y0, z0, x1, y1, z1);
+}
+
+int refresh_render_on(vrt_ctx* c, hipStream_t s, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1,
+                      int64_t z1) {
     x0 = std::max<int64_t>(x0, 0);
     y0 = std::max<int64_t>(y0, 0);
     z0 = std::max<int64_t>(z0, 0);
@@ -309,7 +314,29 @@ int refresh_render(vrt_ctx* c, int64_t x0, int64_t y0, int64_t z0, int64_t x1, i
     y1 = std::min<int64_t>(y1, c->grid.Y);
     z1 = std::min<int64_t>(z1, c->grid.Z);
     if (x1 <= x0 || y1 <= y0 || z1 <= z0) return VRT_OK;
-    return launch_copy_box(c->grid, c->rgrid, x0, y0, z0, x1 - x0, y1 - y0, z1 - z0, c->stream);
+    return launch_copy_box(c->grid, c->rgrid, x0, y0, z0, x1 - x0, y1 - y0, z1 - z0, s);
+}
+
+int remove_sphere_grid(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s) {
+    if (int rc = launch_carve_sphere(c->grid, cx, cy, cz, radius, s)) return rc;
+    const int32_t R = radius + (VRT_DEPTH_FIELD_RADIUS >> 1);
+    if (R > 0) {
+        const int64_t x0 = (int64_t)cx - R, x1 = (int64_t)cx + R, y0 = (int64_t)cy - R, y1 = (int64_t)cy + R,
+                      z0 = (int64_t)cz - R, z1 = (int64_t)cz + R;
+        auto c32 = [](int64_t v) { return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v)); };
+        if (int rc = launch_fix_depth_box(c->grid, c->table, c32(x0), c32(y0), c32(z0), c32(x1), c32(y1), c32(z1),
+                                          cx, cy, cz, R, s))
+            return rc;
+    }
+    return VRT_OK;
+}
+
+int remove_sphere_refresh(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s) {
+    // the carve covers [c - |r|, c + |r|) at most, the repair [c - R, c + R)
+    const int64_t R = (int64_t)radius + (VRT_DEPTH_FIELD_RADIUS >> 1);
+    const int64_t M = std::max<int64_t>(radius < 0 ? -(int64_t)radius : (int64_t)radius, R);
+    return refresh_render_on(c, s, (int64_t)cx - M, (int64_t)cy - M, (int64_t)cz - M, (int64_t)cx + M,
+                             (int64_t)cy + M, (int64_t)cz + M);
 }
 
 int refresh_render_all(vrt_ctx* c) { return refresh_render(c, 0, 0, 0, c->grid.X, c->grid.Y, c->grid.Z); }
@@ -493,7 +520,10 @@ int vrt_upload_region(vrt_ctx* ctx, const int32_t* host_voxels, int32_t x0, int3
         std::swap(y0, y1);
         std::swap(z0, z1);
     }
-    const int64_t xs = std::max<int64_t>(0, x0);
+    // every row starts at getVoxelIndex(x0, y, z), which is -1 (the row is
+    // skipped) for x0 outside the grid
+    if (x0 < 0 || (int64_t)x0 >= (int64_t)g.X) return VRT_OK;
+    const int64_t xs = x0;
     const int64_t xe = std::min<int64_t>((int64_t)g.X - 1, x1);  // inclusive end of the row
     if (xe < xs) return VRT_OK;
     const size_t row_bytes = (size_t)(xe - xs + 1) * sizeof(int32_t);
@@ -512,6 +542,68 @@ int vrt_upload_region(vrt_ctx* ctx, const int32_t* host_voxels, int32_t x0, int3
     // the region can only have gained solid voxels above the old bound
     if (int rc = launch_solid_top(g, xs, ys, zs, xe + 1, ye, ze, false, ctx->stream)) return rc;
     if (int rc = refresh_render(ctx, xs, ys, zs, xe + 1, ye, ze)) return rc;
+    return read_top(ctx);
+}
+
+// updatePartialGeometry, src/render.cpp:204-223, with its float corners and
+// float loops as written: corners swapped when getVoxelIndex of the truncated
+// start exceeds the end's; xLength = int(end.x - start.x) + 1; for float i
+// from start.z while i < end.z (i++), for float j from start.y while j <
+// end.y (j++): the xLength words from getVoxelIndex(int(start.x), int(j),
+// int(i)), when that is a cell, copied as one run (running on into the next
+// row when start.x + xLength passes the row's end, as glBufferSubData does;
+// a run past the end of the grid is not copied: GL rejects it).
+int vrt_upload_region_f(vrt_ctx* ctx, const int32_t* host_voxels, const float start[3], const float end[3]) {
+    if (int rc = check_ctx(ctx)) return rc;
+    if (!host_voxels || !start || !end) return fail(VRT_ERR_INVALID, "null argument");
+    float s0[3] = {start[0], start[1], start[2]}, e0[3] = {end[0], end[1], end[2]};
+    for (int k = 0; k < 3; k++)
+        if (!(std::fabs(s0[k]) < 16777216.0f) || !(std::fabs(e0[k]) < 16777216.0f))
+            return fail(VRT_ERR_INVALID, "corners must be finite and below 2^24 in magnitude (the float loops "
+                                         "of updatePartialGeometry would not end)");
+    const GridDesc& g = ctx->grid;
+    auto index = [&](float x, float y, float z) -> int64_t {
+        const int64_t ix = (int32_t)x, iy = (int32_t)y, iz = (int32_t)z;
+        if (ix >= 0 && iy >= 0 && iz >= 0 && ix < g.X && iy < g.Y && iz < g.Z)
+            return ix + (int64_t)g.X * iy + (int64_t)g.XY * iz;
+        return -1;
+    };
+    if (index(s0[0], s0[1], s0[2]) > index(e0[0], e0[1], e0[2]))
+        for (int k = 0; k < 3; k++) std::swap(s0[k], e0[k]);
+    const int32_t x_len = (int32_t)(e0[0] - s0[0]) + 1;
+    if (x_len <= 0) return VRT_OK;
+    // the int rows the two float loops visit (|values| < 2^24: each i++ moves on)
+    auto rows = [](float from, float to, int64_t n) {
+        std::vector<int64_t> out;
+        for (float v = from; v < to; v++) {
+            const int64_t r = (int32_t)v;
+            if (r >= 0 && r < n) out.push_back(r);
+        }
+        return out;
+    };
+    const std::vector<int64_t> zs = rows(s0[2], e0[2], g.Z), ys = rows(s0[1], e0[1], g.Y);
+    const int64_t xs = (int32_t)s0[0];
+    if (xs < 0 || xs >= (int64_t)g.X || zs.empty() || ys.empty()) return VRT_OK;
+    int64_t lo = INT64_MAX, hi = -1;
+    for (int64_t z : zs)
+        for (int64_t y : ys) {
+            const int64_t off = xs + (int64_t)g.X * y + (int64_t)g.XY * z;
+            if ((uint64_t)(off + x_len) > g.count) continue;
+            VRT_HIP(hipMemcpyAsync(g.vox + off, host_voxels + off, (size_t)x_len * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, ctx->stream));
+            lo = std::min(lo, off);
+            hi = std::max(hi, off + x_len);
+        }
+    if (hi < 0) return VRT_OK;
+    // the cells written lie in whole x-rows of the z-slabs lo..hi-1 touch
+    const int64_t z_lo = lo / (int64_t)g.XY, z_hi = (hi - 1) / (int64_t)g.XY + 1;
+    int64_t y_lo = 0, y_hi = g.Y;
+    if (z_hi - z_lo == 1) {
+        y_lo = (lo % (int64_t)g.XY) / g.X;
+        y_hi = ((hi - 1) % (int64_t)g.XY) / g.X + 1;
+    }
+    if (int rc = launch_solid_top(g, 0, y_lo, z_lo, g.X, y_hi, z_hi, false, ctx->stream)) return rc;
+    if (int rc = refresh_render(ctx, 0, y_lo, z_lo, g.X, y_hi, z_hi)) return rc;
     return read_top(ctx);
 }
 
@@ -566,20 +658,8 @@ int vrt_fix_depth_field_box(vrt_ctx* ctx, int32_t x0, int32_t y0, int32_t z0, in
 int vrt_remove_sphere(vrt_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t radius) {
     if (int rc = check_ctx(ctx)) return rc;
     if (radius < -3 || radius > 4096) return fail(VRT_ERR_INVALID, "radius must be in [-3, 4096]");
-    if (int rc = launch_carve_sphere(ctx->grid, cx, cy, cz, radius, ctx->stream)) return rc;
-    const int32_t R = radius + (VRT_DEPTH_FIELD_RADIUS >> 1);
-    if (R > 0) {
-        const int64_t x0 = (int64_t)cx - R, x1 = (int64_t)cx + R, y0 = (int64_t)cy - R, y1 = (int64_t)cy + R,
-                      z0 = (int64_t)cz - R, z1 = (int64_t)cz + R;
-        auto c32 = [](int64_t v) { return (int32_t)std::max<int64_t>(INT32_MIN, std::min<int64_t>(INT32_MAX, v)); };
-        if (int rc = launch_fix_depth_box(ctx->grid, ctx->table, c32(x0), c32(y0), c32(z0), c32(x1), c32(y1),
-                                          c32(z1), cx, cy, cz, R, ctx->stream))
-            return rc;
-    }
-    // the carve covers [c - |r|, c + |r|) at most, the repair [c - R, c + R)
-    const int64_t M = std::max<int64_t>(radius < 0 ? -(int64_t)radius : (int64_t)radius, R);
-    return refresh_render(ctx, (int64_t)cx - M, (int64_t)cy - M, (int64_t)cz - M, (int64_t)cx + M, (int64_t)cy + M,
-                          (int64_t)cz + M);
+    if (int rc = remove_sphere_grid(ctx, cx, cy, cz, radius, ctx->stream)) return rc;
+    return remove_sphere_refresh(ctx, cx, cy, cz, radius, ctx->stream);
 }
 
 // placeVoxel, src/render.cpp:256-262.

@@ -30,9 +30,7 @@ struct vrt_batch {
 
 namespace vrt {
 
-namespace {
-
-int check_config(const vrt_batch_config& c) {
+int check_batch_config(const vrt_batch_config& c) {
     const bool shape_ok = (c.tile_w == 8 && c.tile_h == 8) || (c.tile_w == 16 && c.tile_h == 4) ||
                           (c.tile_w == 32 && c.tile_h == 2) || (c.tile_w == 16 && c.tile_h == 16) ||
                           (c.tile_w == 64 && c.tile_h == 1);
@@ -41,6 +39,8 @@ int check_config(const vrt_batch_config& c) {
     if (c.waves_per_cu < 0 || c.waves_per_cu > 64) return fail(VRT_ERR_INVALID, "waves_per_cu must be in [0, 64]");
     return VRT_OK;
 }
+
+namespace {
 
 void destroy_plan(vrt_batch* b) {
     if (!b) return;
@@ -57,6 +57,20 @@ void destroy_plan(vrt_batch* b) {
 
 }  // namespace
 
+// One view's descriptor, its tiles starting at first_tile of the launch.
+int make_view_desc(const vrt_view& v, const vrt_batch_config& cfg, int64_t first_tile, ViewDesc& d) {
+    if (int rc = check_frame(v.frame)) return rc;
+    if (int rc = check_target(v.target)) return rc;
+    std::memset(&d, 0, sizeof(d));
+    d.frame = make_frame_desc(v.frame, v.target);
+    d.target = v.target;
+    d.tiles_x = (v.target.width + cfg.tile_w - 1) / cfg.tile_w;
+    const int32_t tiles_y = (v.target.height + cfg.tile_h - 1) / cfg.tile_h;
+    d.tiles = d.tiles_x * tiles_y;
+    d.first_tile = first_tile;
+    return VRT_OK;
+}
+
 // Fills descriptors (largest view first) and the tile total.
 int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& cfg, std::vector<ViewDesc>& out,
                      int64_t& total_tiles) {
@@ -70,20 +84,12 @@ int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& c
     out.clear();
     total_tiles = 0;
     for (int32_t i : order) {
-        if (int rc = check_frame(views[i].frame)) return rc;
-        if (int rc = check_target(views[i].target)) return rc;
         ViewDesc d;
-        std::memset(&d, 0, sizeof(d));
-        d.frame = make_frame_desc(views[i].frame, views[i].target);
-        d.target = views[i].target;
-        d.tiles_x = (views[i].target.width + cfg.tile_w - 1) / cfg.tile_w;
-        const int32_t tiles_y = (views[i].target.height + cfg.tile_h - 1) / cfg.tile_h;
-        d.tiles = d.tiles_x * tiles_y;
-        d.first_tile = total_tiles;
+        if (int rc = make_view_desc(views[i], cfg, total_tiles, d)) return rc;
         total_tiles += d.tiles;
         out.push_back(d);
     }
-    if (total_tiles > 0xffffffffll - 65536) return fail(VRT_ERR_INVALID, "too many tiles in one batch");
+    if (total_tiles > kMaxBatchTiles) return fail(VRT_ERR_INVALID, "too many tiles in one batch");
     return VRT_OK;
 }
 
@@ -117,7 +123,7 @@ int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_b
     *out = nullptr;
     VRT_HIP(hipSetDevice(ctx->device));
     vrt_batch_config cfg = cfg_in ? *cfg_in : ctx->tuned_cfg;
-    if (int rc = check_config(cfg)) return rc;
+    if (int rc = check_batch_config(cfg)) return rc;
     vrt_batch* b = new vrt_batch();
     b->ctx = ctx;
     b->cfg = cfg;

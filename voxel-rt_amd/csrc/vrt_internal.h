@@ -92,6 +92,8 @@ constexpr int kCounterStride = 16;  // u64 per slot
 
 // Queue of the persistent batch kernel: 8 counters (one per XCD), 128 B apart.
 constexpr size_t kQueueBytes = 8 * 128;
+// Tiles one multi-view launch may hold (queue counters are 32-bit).
+constexpr int64_t kMaxBatchTiles = 0xffffffffll - 65536;
 
 // Per-view work description for the persistent batch kernel.
 struct ViewDesc {
@@ -212,6 +214,13 @@ int solid_top_full(vrt_ctx* c);  // recomputes *grid.top over the whole grid
 // grid into the render copy, on the context's stream.
 int refresh_render(vrt_ctx* c, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1, int64_t z1);
 int refresh_render_all(vrt_ctx* c);
+int refresh_render_on(vrt_ctx* c, hipStream_t s, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1,
+                      int64_t z1);
+// removeSphere in two parts on stream s: the carve and depth-field repair of
+// the grid (render kernels never read it), then the copy of the edited box
+// into the render copy (what in-flight renders must not see).
+int remove_sphere_grid(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s);
+int remove_sphere_refresh(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s);
 }  // namespace vrt
 
 namespace vrt {
@@ -219,5 +228,15 @@ int world_generate_device(const GridDesc& g, const vrt_world_params& p, hipStrea
 FrameDesc make_frame_desc(const vrt_frame& f, const vrt_target& target);
 int32_t plain_row_rotation(const vrt_frame& f, const vrt_target& t, int32_t top, bool primary_cutoff);
 int check_frame(const vrt_frame& f);
+// Tile shape in {8x8, 16x4, 32x2, 16x16, 64x1}, chunk_tiles in [0, 4096],
+// waves_per_cu in [0, 64]; the batch kernels assume all three.
+int check_batch_config(const vrt_batch_config& c);
+// One view's descriptor for a multi-view launch, its tiles numbered from
+// first_tile; checks the frame and target.
+struct ViewDesc;
+int make_view_desc(const vrt_view& v, const vrt_batch_config& cfg, int64_t first_tile, ViewDesc& d);
+// Workgroups of a multi-view launch: one per four tiles when the hardware
+// schedules (chunk_tiles 0), else persistent workgroups from occupancy.
+int persistent_workgroups(vrt_ctx* c, const vrt_batch_config& cfg, int64_t total_tiles, int32_t* out);
 int check_target(const vrt_target& t);
 }  // namespace vrt

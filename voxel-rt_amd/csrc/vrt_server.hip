@@ -3,16 +3,24 @@
 // are admitted by framebuffer memory into continuous batches.
 //
 // Ordering.  Requests and edits form one FIFO.  A request sees every edit
-// queued before it and none after: an edit is launched on the edit stream
-// after waiting (hipStreamWaitEvent) for every in-flight batch, and every
-// later batch waits for that edit's event.  The host never blocks on an edit.
+// queued before it and none after.  An edit runs on the edit stream in two
+// parts: the carve and depth-field repair of the grid start at once (render
+// kernels read only the padded render copy, never the grid), and the copy of
+// the edited box into the render copy waits (hipStreamWaitEvent) for every
+// batch in flight, which must not see it; every later batch waits for that
+// copy.  The host never blocks on an edit.
 //
 // Continuous batching.  Each pump admits queued requests, oldest first, while
 // their frame buffers fit in the budget and a batch slot (stream) is free, and
-// launches them as one persistent multi-view launch on that slot's stream.
-// Requests arriving while earlier batches run go into the next slot, so their
-// work overlaps the tail of the running batches on the device.  A completed
+// launches them as one multi-view launch on that slot's stream.  Requests
+// arriving while earlier batches run go into the next slot, so their work
+// overlaps the tail of the running batches on the device.  A completed
 // request keeps its buffers (and its share of the budget) until released.
+//
+// Memory.  The budget is reserved as one device arena at creation; request
+// buffers are carved from it first-fit and returned on release, so serving
+// never calls hipMalloc or hipFree (hipFree waits for the whole device,
+// which would stall the batches running on other slots).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,7 +34,6 @@
 namespace vrt {
 int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& cfg, std::vector<ViewDesc>& out,
                      int64_t& total_tiles);
-int persistent_workgroups(vrt_ctx* c, const vrt_batch_config& cfg, int64_t total_tiles, int32_t* out);
 }  // namespace vrt
 
 namespace {
@@ -36,10 +43,12 @@ struct Request {
     vrt_frame frame{};
     int32_t width = 0, height = 0;
     uint64_t bytes = 0;
-    double t_submit = 0, t_start = 0, t_done = 0;
+    double t_submit = 0, t_start = 0, t_exec = 0, t_gpu_done = 0, t_done = 0;
     uint64_t world_version = 0;
     vrt_target target{};
-    void* mem = nullptr;
+    bool has_mem = false;
+    uint64_t offset = 0;  // into the arena
+    uint64_t span = 0;    // bytes carved from the arena
     bool done = false;
 };
 
@@ -51,12 +60,47 @@ struct Item {  // FIFO entry: a request or an edit
 
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
+    hipEvent_t exec = nullptr;  // the batch reached the device (timing)
+    hipEvent_t done = nullptr;  // the batch ended (timing)
     vrt::ViewDesc* views_dev = nullptr;
     vrt::ViewDesc* views_host = nullptr;  // pinned staging
     unsigned int* queue = nullptr;
     bool busy = false;
     std::vector<uint64_t> ids;
+};
+
+// First-fit sub-allocation of one device range; free blocks coalesce.
+struct Arena {
+    uint8_t* base = nullptr;
+    uint64_t size = 0;
+    std::map<uint64_t, uint64_t> free_blocks;  // offset -> bytes
+
+    bool take(uint64_t bytes, uint64_t* off) {
+        for (auto it = free_blocks.begin(); it != free_blocks.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const uint64_t rest = it->second - bytes, at = it->first + bytes;
+            free_blocks.erase(it);
+            if (rest) free_blocks.emplace(at, rest);
+            return true;
+        }
+        return false;
+    }
+    void give(uint64_t off, uint64_t bytes) {
+        auto it = free_blocks.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_blocks.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_blocks.erase(next);
+        }
+        if (it != free_blocks.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_blocks.erase(it);
+            }
+        }
+    }
 };
 
 }  // namespace
@@ -70,6 +114,8 @@ struct vrt_server {
     uint64_t version_applied = 0;  // edits launched on the device
     hipStream_t edit_stream = nullptr;
     hipEvent_t edit_done = nullptr;
+    hipEvent_t ref = nullptr;  // device timestamp of host time t_ref
+    double t_ref = 0;
     bool have_edit = false;
     std::deque<Item> fifo;
     std::map<uint64_t, Request> reqs;
@@ -77,14 +123,13 @@ struct vrt_server {
     std::deque<uint64_t> completed;
     uint64_t bytes_in_use = 0;
     int32_t queued = 0;
-    // Released frame buffers kept for reuse, keyed by size: hipFree would
-    // synchronise the device and stall batches running on other slots.
-    std::multimap<uint64_t, void*> pool;
-    uint64_t pool_bytes = 0;
+    Arena arena;
 };
 
 namespace vrt {
 namespace {
+
+uint64_t align256(uint64_t v) { return (v + 255) & ~(uint64_t)255; }
 
 uint64_t frame_bytes(const vrt_server_config& c, int32_t w, int32_t h) {
     const uint64_t px = (uint64_t)w * (uint64_t)h;
@@ -94,27 +139,16 @@ uint64_t frame_bytes(const vrt_server_config& c, int32_t w, int32_t h) {
     return px * per + 6 * 256;           // each buffer 256-byte aligned
 }
 
-uint64_t align256(uint64_t v) { return (v + 255) & ~(uint64_t)255; }
-
-int alloc_target(vrt_server* s, Request& r) {
+// Carves the request's buffers out of the arena; false when no free block is
+// large enough (the caller then waits for releases, as for the budget).
+bool alloc_target(vrt_server* s, Request& r) {
     const uint64_t px = (uint64_t)r.width * (uint64_t)r.height;
-    void* mem = nullptr;
-    auto hit = s->pool.find(r.bytes);
-    if (hit != s->pool.end()) {
-        mem = hit->second;
-        s->pool.erase(hit);
-        s->pool_bytes -= r.bytes;
-    } else {
-        // keep cached plus live buffers within the budget
-        while (!s->pool.empty() && s->pool_bytes + s->bytes_in_use + r.bytes > s->cfg.framebuffer_budget) {
-            auto big = std::prev(s->pool.end());
-            (void)hipFree(big->second);
-            s->pool_bytes -= big->first;
-            s->pool.erase(big);
-        }
-        VRT_HIP(hipMalloc(&mem, r.bytes));
-    }
-    uint8_t* p = (uint8_t*)mem;
+    uint64_t span = align256(px * 3);
+    if (s->cfg.want_depth) span += align256(px * 4);
+    if (s->cfg.want_aux) span += align256(px * 8) + align256(px) + 2 * align256(px * 4) + align256(px * 4);
+    uint64_t off = 0;
+    if (!s->arena.take(span, &off)) return false;
+    uint8_t* p = s->arena.base + off;
     std::memset(&r.target, 0, sizeof(r.target));
     r.target.width = r.width;
     r.target.height = r.height;
@@ -135,11 +169,23 @@ int alloc_target(vrt_server* s, Request& r) {
         p += align256(px * 4);
         r.target.steps = (uint32_t*)p;
     }
-    r.mem = mem;
-    return VRT_OK;
+    r.has_mem = true;
+    r.offset = off;
+    r.span = span;
+    return true;
 }
 
 double clock_of(vrt_server* s) { return now_seconds() - s->t0; }
+
+// Device time of a completed event on the server's clock.
+double device_time(vrt_server* s, hipEvent_t e) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, s->ref, e) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0;
+    }
+    return s->t_ref + 1e-3 * (double)ms;
+}
 
 int retire(vrt_server* s) {
     for (auto& sl : s->slots) {
@@ -148,9 +194,12 @@ int retire(vrt_server* s) {
         if (e == hipErrorNotReady) continue;
         if (e != hipSuccess) return hip_fail(e, "batch completion");
         const double t = clock_of(s);
+        const double t_exec = device_time(s, sl.exec), t_gpu = device_time(s, sl.done);
         for (uint64_t id : sl.ids) {
             auto it = s->reqs.find(id);
             if (it == s->reqs.end()) continue;
+            it->second.t_exec = t_exec;
+            it->second.t_gpu_done = t_gpu;
             it->second.t_done = t;
             it->second.done = true;
             s->completed.push_back(id);
@@ -163,14 +212,12 @@ int retire(vrt_server* s) {
 
 int apply_edit(vrt_server* s, const Item& it) {
     vrt_ctx* c = s->ctx;
-    VRT_HIP(hipStreamWaitEvent(s->edit_stream, s->edit_done, 0));
+    // carve + repair of the grid: no render reads it, so no wait
+    if (int rc = remove_sphere_grid(c, it.cx, it.cy, it.cz, it.r, s->edit_stream)) return rc;
+    // the render copy changes only after every batch that must not see the edit
     for (auto& sl : s->slots)
         if (sl.busy) VRT_HIP(hipStreamWaitEvent(s->edit_stream, sl.done, 0));
-    hipStream_t keep = c->stream;
-    c->stream = s->edit_stream;
-    int rc = vrt_remove_sphere(c, it.cx, it.cy, it.cz, it.r);
-    c->stream = keep;
-    if (rc) return rc;
+    if (int rc = remove_sphere_refresh(c, it.cx, it.cy, it.cz, it.r, s->edit_stream)) return rc;
     VRT_HIP(hipEventRecord(s->edit_done, s->edit_stream));
     s->have_edit = true;
     s->version_applied++;
@@ -195,6 +242,7 @@ int launch_slot(vrt_server* s, Slot& sl, std::vector<uint64_t>& ids) {
     // finished (slot not busy), so the async copy never races a rewrite.
     std::memcpy(sl.views_host, descs.data(), descs.size() * sizeof(ViewDesc));
     if (s->have_edit) VRT_HIP(hipStreamWaitEvent(sl.stream, s->edit_done, 0));
+    VRT_HIP(hipEventRecord(sl.exec, sl.stream));
     VRT_HIP(hipMemcpyAsync(sl.views_dev, sl.views_host, descs.size() * sizeof(ViewDesc), hipMemcpyHostToDevice,
                            sl.stream));
     if (cfg.chunk_tiles)
@@ -235,10 +283,7 @@ int admit(vrt_server* s, int* launched) {
         while (!s->fifo.empty() && !s->fifo.front().is_edit && (int32_t)ids.size() < s->cfg.max_batch_views) {
             Request& r = s->reqs[s->fifo.front().id];
             if (s->bytes_in_use + r.bytes > s->cfg.framebuffer_budget) break;
-            if (int rc = alloc_target(s, r)) {
-                if (ids.empty()) return rc;
-                break;  // device memory short: launch what we have
-            }
+            if (!alloc_target(s, r)) break;  // no block large enough: wait for releases
             s->bytes_in_use += r.bytes;
             ids.push_back(r.id);
             s->fifo.pop_front();
@@ -253,19 +298,20 @@ int admit(vrt_server* s, int* launched) {
 void destroy_server(vrt_server* s) {
     if (!s) return;
     if (s->ctx) (void)hipSetDevice(s->ctx->device);
-    for (auto& sl : s->slots) {
+    for (auto& sl : s->slots)
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (s->edit_stream) (void)hipStreamSynchronize(s->edit_stream);
+    for (auto& sl : s->slots) {
+        if (sl.exec) (void)hipEventDestroy(sl.exec);
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.views_dev) (void)hipFree(sl.views_dev);
         if (sl.views_host) (void)hipHostFree(sl.views_host);
         if (sl.queue) (void)hipFree(sl.queue);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
-    if (s->edit_stream) (void)hipStreamSynchronize(s->edit_stream);
-    for (auto& kv : s->reqs)
-        if (kv.second.mem) (void)hipFree(kv.second.mem);
-    for (auto& kv : s->pool) (void)hipFree(kv.second);
+    if (s->arena.base) (void)hipFree(s->arena.base);
     if (s->edit_done) (void)hipEventDestroy(s->edit_done);
+    if (s->ref) (void)hipEventDestroy(s->ref);
     if (s->edit_stream) (void)hipStreamDestroy(s->edit_stream);
     delete s;
 }
@@ -283,6 +329,9 @@ int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** o
     if (cfg->max_batch_views < 1 || cfg->max_batch_views > 4096 || cfg->streams < 1 || cfg->streams > 16 ||
         cfg->framebuffer_budget == 0)
         return fail(VRT_ERR_INVALID, "server config out of range");
+    // tile_w == 0 takes the context's default (already checked when tuned)
+    if (cfg->batch.tile_w != 0)
+        if (int rc = check_batch_config(cfg->batch)) return rc;
     VRT_HIP(hipSetDevice(ctx->device));
     vrt_server* s = new vrt_server();
     s->ctx = ctx;
@@ -293,20 +342,33 @@ int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** o
         return rc;
     };
     hipError_t e;
+    s->arena.size = align256(cfg->framebuffer_budget);
+    if ((e = hipMalloc(&s->arena.base, s->arena.size)) != hipSuccess) {
+        s->arena.base = nullptr;
+        return bail(hip_fail(e, "hipMalloc of the framebuffer budget"));
+    }
+    s->arena.free_blocks.emplace(0, s->arena.size);
     if ((e = hipStreamCreateWithFlags(&s->edit_stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(hip_fail(e, "hipStreamCreate"));
     if ((e = hipEventCreateWithFlags(&s->edit_done, hipEventDisableTiming)) != hipSuccess)
         return bail(hip_fail(e, "hipEventCreate"));
+    if ((e = hipEventCreate(&s->ref)) != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
     // edits and batches must also follow whatever the context queued before
     if ((e = hipEventRecord(s->edit_done, ctx->stream)) != hipSuccess) return bail(hip_fail(e, "hipEventRecord"));
+    if ((e = hipStreamWaitEvent(s->edit_stream, s->edit_done, 0)) != hipSuccess)
+        return bail(hip_fail(e, "hipStreamWaitEvent"));
     s->have_edit = true;
+    // the clock anchor: an event the host waits for, stamped with the host time
+    if ((e = hipEventRecord(s->ref, s->edit_stream)) != hipSuccess) return bail(hip_fail(e, "hipEventRecord"));
+    if ((e = hipEventSynchronize(s->ref)) != hipSuccess) return bail(hip_fail(e, "hipEventSynchronize"));
+    s->t_ref = clock_of(s);
     s->slots.resize(cfg->streams);
     const size_t vbytes = (size_t)cfg->max_batch_views * sizeof(ViewDesc);
     for (auto& sl : s->slots) {
         if ((e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess)
             return bail(hip_fail(e, "hipStreamCreate"));
-        if ((e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming)) != hipSuccess)
-            return bail(hip_fail(e, "hipEventCreate"));
+        if ((e = hipEventCreate(&sl.exec)) != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
+        if ((e = hipEventCreate(&sl.done)) != hipSuccess) return bail(hip_fail(e, "hipEventCreate"));
         if ((e = hipMalloc(&sl.views_dev, vbytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc"));
         if ((e = hipHostMalloc(&sl.views_host, vbytes, hipHostMallocDefault)) != hipSuccess)
             return bail(hip_fail(e, "hipHostMalloc"));
@@ -410,6 +472,8 @@ int vrt_server_poll(vrt_server* s, vrt_result* out, int32_t max, int32_t* n) {
         o.height = r.height;
         o.t_submit = r.t_submit;
         o.t_start = r.t_start;
+        o.t_exec = r.t_exec;
+        o.t_gpu_done = r.t_gpu_done;
         o.t_done = r.t_done;
         o.target = r.target;
     }
@@ -420,10 +484,9 @@ int vrt_server_poll(vrt_server* s, vrt_result* out, int32_t max, int32_t* n) {
 int vrt_server_release(vrt_server* s, uint64_t id) {
     if (!s) return fail(VRT_ERR_INVALID, "invalid server");
     auto it = s->reqs.find(id);
-    if (it == s->reqs.end() || it->second.mem == nullptr || !it->second.done)
+    if (it == s->reqs.end() || !it->second.has_mem || !it->second.done)
         return fail(VRT_ERR_NOTFOUND, "no completed request with this id");
-    s->pool.emplace(it->second.bytes, it->second.mem);
-    s->pool_bytes += it->second.bytes;
+    s->arena.give(it->second.offset, it->second.span);
     s->bytes_in_use -= it->second.bytes;
     s->reqs.erase(it);
     return VRT_OK;

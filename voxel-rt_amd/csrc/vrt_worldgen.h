@@ -1,6 +1,6 @@
-// vrt_worldgen.h — the synthetic world as a pure per-column function, shared
-// by the host generator (vrt_world.cpp) and the device generator
-// (vrt_world.hip) so both write the same words.  Integer arithmetic only.
+// vrt_worldgen.h — the synthetic world as a pure per-column function, used by
+// both generators in vrt_world.hip (host threads over z-slabs and one device
+// thread per column) so both write the same words.  Integer arithmetic only.
 //
 // World: value-noise heightfield (octaves of lattice noise, amplitudes
 // 2^(octaves-1) .. 1, smoothstep interpolation in Q16) giving heights in
@@ -13,11 +13,7 @@
 
 #include <cstdint>
 
-#if defined(__HIPCC__)
 #define VRT_HD __host__ __device__ __forceinline__
-#else
-#define VRT_HD inline
-#endif
 
 namespace vrt {
 namespace wg {

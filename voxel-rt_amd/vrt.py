@@ -68,7 +68,8 @@ class vrt_server_config(C.Structure):
 
 class vrt_result(C.Structure):
     _fields_ = [("id", C.c_uint64), ("world_version", C.c_uint64), ("width", C.c_int32), ("height", C.c_int32),
-                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("t_submit", C.c_double), ("t_start", C.c_double), ("t_exec", C.c_double),
+                ("t_gpu_done", C.c_double), ("t_done", C.c_double),
                 ("target", vrt_target)]
 
 
@@ -100,6 +101,7 @@ SIGNATURES = {
     "vrt_upload": (C.c_int, [P, P]),
     "vrt_upload_device": (C.c_int, [P, P]),
     "vrt_upload_region": (C.c_int, [P, P, I32, I32, I32, I32, I32, I32]),
+    "vrt_upload_region_f": (C.c_int, [P, P, FP, FP]),
     "vrt_download": (C.c_int, [P, P]),
     "vrt_depth_table": (C.c_int, [FP, C.POINTER(I32), I32, C.POINTER(I32)]),
     "vrt_build_depth_field": (C.c_int, [P]),
@@ -399,6 +401,13 @@ class Context:
     def upload_region(self, world, lo, hi):
         w = np.ascontiguousarray(world, dtype=np.int32)
         call("vrt_upload_region", self.h, w.ctypes.data_as(P), *[int(v) for v in lo], *[int(v) for v in hi])
+
+    def upload_region_f(self, world, start, end):
+        """updatePartialGeometry with the reference's float corners."""
+        w = np.ascontiguousarray(world, dtype=np.int32)
+        s = (C.c_float * 3)(*[float(v) for v in start])
+        e = (C.c_float * 3)(*[float(v) for v in end])
+        call("vrt_upload_region_f", self.h, w.ctypes.data_as(P), s, e)
 
     def download(self):
         out = np.empty(self.dims[0] * self.dims[1] * self.dims[2], dtype=np.int32)
