@@ -124,7 +124,7 @@ def test_reference_world_startup_view_matches_oracle(gpu, orc, lights):
     ctx.draw(tgt, frame=f)
     g = tgt.download()
     g.update(ctx.stats(reset=True))
-    r = orc.render(wdf, dims, f, 800, 600)
+    r = orc.LazyWorld(world, dims).render(f, 800, 600)  # from the raw world, not the GPU's grid
     assert_frames_match(g, r, "reference start-up view, %d lights" % lights)
     assert (r["hit"] >= 0).mean() > 0.3
     if lights:

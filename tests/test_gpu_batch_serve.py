@@ -45,7 +45,8 @@ def scene(gpu, orc):
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
     ctx.build_depth_field()
-    wdf = ctx.download()
+    wdf = orc.depth_field(sc.world, sc.dims)  # the oracle's own grid
+    assert np.array_equal(ctx.download(), wdf)
     yield sc, ctx, wdf
     ctx.close()
 
@@ -161,7 +162,7 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
     ctx.build_depth_field()
-    wdf = ctx.download()
+    wdf = orc.depth_field(sc.world, sc.dims)  # the oracle replays the log on its own grid
     views = small_views(gpu, sc, 14, seed=5)
     rng = np.random.default_rng(6)
     s = C.c_void_p()
@@ -182,6 +183,8 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
             gpu.call("vrt_server_pump", s, 0)  # admit part of the stream while the rest arrives
     got = drain(gpu, s, len(views))
     assert len(got) == len(views)
+    for rid, (_, _, r) in got.items():  # device times sit between the host's (1 ms for the clock anchor)
+        assert r.t_start <= r.t_exec + 1e-3 and r.t_exec <= r.t_gpu_done and r.t_gpu_done <= r.t_done + 1e-3, rid
     # the oracle replays the same log
     world = wdf.copy()
     version = 0
@@ -325,6 +328,14 @@ def test_failures_are_reported_not_fatal(gpu):
     arr[0].target = t.struct()
     b = C.c_void_p()
     assert gpu.lib().vrt_batch_create(ctx.h, arr, 1, C.byref(cfg), C.byref(b)) == gpu.VRT_ERR_INVALID
+    # a server's batch config is checked the same way (8x4 would shade no row,
+    # 128x1 has no whole row per wave pass)
+    for tw, th, ch in ((8, 4, 1), (128, 1, 0), (8, 8, -1)):
+        sc_cfg = server_config(gpu, 1 << 24)
+        sc_cfg.batch = gpu.vrt_batch_config(tw, th, ch, 0)
+        srv = C.c_void_p()
+        assert gpu.lib().vrt_server_create(ctx.h, C.byref(sc_cfg), C.byref(srv)) == gpu.VRT_ERR_INVALID
+        assert not srv.value
     # still usable
     ctx.draw(t, frame=f)
     assert t.download()["rgb"].shape == (16, 16, 3)

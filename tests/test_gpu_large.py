@@ -31,28 +31,51 @@ def compare_sample(g, r, pix, label):
 
 
 def test_large_world_4k_shaded_and_depth_view(gpu, orc):
+    """The 8 GiB world: the device generator equals the host generator; the
+    GPU depth field equals the oracle's on whole slabs through the camera and
+    at the surface plus random voxels; sampled pixels (random and the longest
+    walks) of both modes equal the oracle's, which renders from the raw world
+    with its depth-field words computed on demand (never from the GPU's grid)."""
     sc = gpu.make_scene("large4k", seed=3)
     X, Y, Z = sc.dims
     print("large world generated", flush=True)
+    # about 35% of the cells below the terrain surface are carved out (caves)
+    rng = np.random.default_rng(4)
+    cols = rng.integers(0, [X, Z], size=(4000, 2))
+    below = empty = 0
+    for x, z in cols:
+        h = gpu.surface(sc.params, int(x), int(z))
+        col = sc.world[int(x) + X * np.arange(h, dtype=np.int64) + X * Y * int(z)]
+        below += h
+        empty += int((col < 0).sum())
+    frac = empty / below
+    print("empty below the surface: %.3f" % frac, flush=True)
+    assert 0.30 <= frac <= 0.40, frac
     ctx = gpu.Context(sc.dims)
     ctx.generate(sc.params)  # device generator
     dev = ctx.download()
     assert np.array_equal(dev, sc.world), "device world differs from host world"
     del dev
     ctx.build_depth_field()
-    wdf = ctx.download()
-    rng = np.random.default_rng(4)
-    n = 100000
-    xyz = np.stack([rng.integers(0, X, n), rng.integers(0, 128, n), rng.integers(0, Z, n)], 1).astype(np.int32)
-    idx = xyz[:, 0].astype(np.int64) + X * xyz[:, 1].astype(np.int64) + X * Y * xyz[:, 2].astype(np.int64)
-    exp = orc.fix_depth_values(sc.world, sc.dims, xyz)
-    assert np.array_equal(wdf[idx], exp), "depth field differs on sampled voxels"
+    lazy = orc.LazyWorld(sc.world, sc.dims)
+    gdf = ctx.download()
+    # whole slabs: the z-slab and the x-row band through the camera, the
+    # y-slab just above the camera's surface height, and 200k random voxels
+    cz, cy = int(sc.frame.cam_pos[2]), int(sc.frame.cam_pos[1])
+    zs = X * Y * cz + np.arange(X * Y, dtype=np.int64)
+    ys = (np.arange(X, dtype=np.int64)[None, :] + X * cy + X * Y * np.arange(Z, dtype=np.int64)[:, None]).ravel()
+    rnd = rng.integers(0, X * Y * Z, 200000)
+    for name, idx in (("z-slab %d" % cz, zs), ("y-slab %d" % cy, ys), ("random", rnd)):
+        exp = lazy.words(idx)
+        bad = np.nonzero(gdf[idx] != exp)[0]
+        assert bad.size == 0, "depth field differs on the %s at %d voxels" % (name, bad.size)
+    del gdf
     print("depth field checked", flush=True)
     for mode in (0, 1):
         sc.frame.view_depth_field = mode
         g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
         pix = sampled_pixels(g, 3000, 300, seed=mode)
-        r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height, pixels=pix)
+        r = lazy.render(sc.frame, sc.width, sc.height, pixels=pix)
         compare_sample(g, r, pix, "large world mode %d" % mode)
         # the grazing camera drives walks far beyond the 384 cap of the reference
         assert int(g["steps"].max()) > 100

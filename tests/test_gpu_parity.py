@@ -217,39 +217,97 @@ def test_place_destroy_and_region_upload(gpu, orc):
     ev = expect.reshape(Z, Y, X)
     ev[15:30, 5:20, 15:36] = hv[15:30, 5:20, 15:36]
     assert np.array_equal(ctx.download(), expect)
+    # start x outside the grid: getVoxelIndex(-1, y, z) is -1 for every row,
+    # so updatePartialGeometry uploads nothing
+    host2 = host.copy()
+    host2[host2 >= 0] = 0x010203
+    ctx.upload_region(host2, (-1, 0, 0), (5, 3, 3))
+    assert np.array_equal(ctx.download(), expect)
     ctx.close()
 
 
-def test_frame1080_matches_oracle(gpu, orc):
-    """The 1080p drop-in frame: GPU-built depth field checked on a sample of
-    voxels, then the whole frame against the oracle on that world."""
+def test_region_upload_with_float_corners(gpu):
+    """updatePartialGeometry with float corners (src/render.cpp:204-223): the
+    float loops' rows and the run length int(end.x - start.x) + 1, on cases
+    whose rows follow from the rules by hand."""
+    X, Y, Z = 24, 8, 6
+    rng = np.random.default_rng(5)
+    host = rng.integers(0, 1 << 24, X * Y * Z).astype(np.int32)
+    f32 = np.float32
+
+    def run(start, end):
+        ctx = gpu.Context((X, Y, Z))  # every cell -1 until uploaded
+        ctx.upload_region_f(host, start, end)
+        got = ctx.download().reshape(Z, Y, X)
+        ctx.close()
+        return got
+
+    def expect(cells):
+        e = np.full((Z, Y, X), -1, np.int32)
+        h = host.reshape(Z, Y, X)
+        for (x, y, z) in cells:
+            e[z, y, x] = h[z, y, x]
+        return e
+
+    # i = 0.5, 1.5, 2.5 (< 2.9) -> z 0..2; j = 0.5, 1.5 (< 2.2) -> y 0, 1; run int(3.2) + 1 = 4 from x 0
+    got = run((0.5, 0.5, 0.5), (3.7, 2.2, 2.9))
+    assert np.array_equal(got, expect([(x, y, z) for z in range(3) for y in range(2) for x in range(4)]))
+    # j = 0.99999994 -> row 0; j + 1 = 1.99999994 rounds (to even) to 2.0 -> row 2; 3.0 -> row 3: row 1 skipped
+    j0 = float(np.nextafter(f32(1.0), f32(0.0)))
+    got = run((2.0, j0, 1.0), (4.0, 3.5, 2.0))
+    assert np.array_equal(got, expect([(x, y, 1) for y in (0, 2, 3) for x in range(2, 5)]))
+    # corners given high to low: swapped first (getVoxelIndex of start above end's)
+    got = run((4.0, 3.5, 2.0), (2.0, 0.0, 1.0))
+    assert np.array_equal(got, expect([(x, y, 1) for y in range(4) for x in range(2, 5)]))
+    # int(start.x) = -1: every row's getVoxelIndex is -1, nothing is uploaded
+    assert np.array_equal(run((-1.5, 0.0, 0.0), (5.0, 3.0, 3.0)), expect([]))
+    # start and end both outside (no swap): rows y = 0, 1 of z = 0 (j = -1 skipped), each a run of
+    # int(3.0) + 1 = 4 words from x = X - 2, which runs on into the next row
+    got = run((X - 2.0, -1.0, 0.0), (X + 1.0, 2.0, 1.0))
+    cells = [(X - 2, 0, 0), (X - 1, 0, 0), (0, 1, 0), (1, 1, 0), (X - 2, 1, 0), (X - 1, 1, 0), (0, 2, 0), (1, 2, 0)]
+    assert np.array_equal(got, expect(cells))
+    # corners whose float loops would not end are refused
+    ctx = gpu.Context((X, Y, Z))
+    w = np.ascontiguousarray(host)
+    s = (gpu.C.c_float * 3)(0.0, 0.0, 0.0)
+    e = (gpu.C.c_float * 3)(3.0, 3.0, 3.0e7)
+    assert gpu.lib().vrt_upload_region_f(ctx.h, w.ctypes.data_as(gpu.P), s, e) == gpu.VRT_ERR_INVALID
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def frame1080(gpu, orc):
+    """The 1080p drop-in world on the GPU, and the oracle's own depth field of
+    it (computeDepthField over every voxel on the CPU), which must equal the
+    GPU-built one word for word; the frame tests render the oracle from the
+    oracle's grid, so nothing the GPU built enters the expected frames."""
     sc = gpu.make_scene("frame1080", seed=1)
-    X, Y, Z = sc.dims
     ctx = gpu.Context(sc.dims)
     ctx.upload(sc.world)
     ctx.build_depth_field()
-    wdf = ctx.download()
-    rng = np.random.default_rng(2)
-    n = 200000
-    xyz = np.stack([rng.integers(0, X, n), rng.integers(0, Y, n), rng.integers(0, Z, n)], 1).astype(np.int32)
-    # include every voxel of two full columns' neighbourhood slabs near the surface
-    exp = orc.fix_depth_values(sc.world, sc.dims, xyz)
-    idx = xyz[:, 0].astype(np.int64) + X * xyz[:, 1] + X * Y * xyz[:, 2].astype(np.int64)
-    assert np.array_equal(wdf[idx], exp), "depth field differs on %d sampled voxels" % int((wdf[idx] != exp).sum())
-    g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
-    r = orc.render(wdf, sc.dims, sc.frame, sc.width, sc.height)
-    assert_frames_match(g, r, "frame1080")
+    print("1080p world on the GPU; oracle depth field of all %d voxels" % sc.world.size, flush=True)
+    odf = orc.depth_field(sc.world, sc.dims)
+    gdf = ctx.download()
+    bad = np.nonzero(gdf != odf)[0]
+    assert bad.size == 0, "GPU depth field differs from the oracle's at %d voxels, first %d" % (bad.size, bad[0])
+    del gdf
+    yield sc, ctx, odf
     ctx.close()
 
 
-def test_frame1080_more_cameras_match_oracle(gpu, orc):
+def test_frame1080_matches_oracle(gpu, orc, frame1080):
+    """The 1080p drop-in frame, every pixel, against the oracle rendering its
+    own depth field of the raw world."""
+    sc, ctx, odf = frame1080
+    g = gpu.render_on_gpu(sc, aux=True, ctx=ctx)
+    r = orc.render(odf, sc.dims, sc.frame, sc.width, sc.height)
+    assert_frames_match(g, r, "frame1080")
+
+
+def test_frame1080_more_cameras_match_oracle(gpu, orc, frame1080):
     """Six more 1080p cameras over the C2 world (lights clustered around the
     camera so that many are in range, one depth-view frame), cutoff on."""
-    sc = gpu.make_scene("frame1080", seed=1)
-    ctx = gpu.Context(sc.dims)
-    ctx.upload(sc.world)
-    ctx.build_depth_field()
-    wdf = ctx.download()
+    sc, ctx, odf = frame1080
     rng = np.random.default_rng(31)
     for k in range(6):
         f = gpu.make_frame(sc.params, sc.dims, sc.world, rng, 1920, 1080, 16, 384, light_spread=48.0,
@@ -259,22 +317,26 @@ def test_frame1080_more_cameras_match_oracle(gpu, orc):
         ctx.draw(t, frame=f)
         g = t.download()
         g.update(ctx.stats(reset=True))
-        r = orc.render(wdf, sc.dims, f, 1920, 1080)
+        r = orc.render(odf, sc.dims, f, 1920, 1080)
         assert_frames_match(g, r, "1080p camera %d" % k)
-    ctx.close()
 
 
 @pytest.mark.parametrize("case", ["frame1080_cameras", "terrain128"])
-def test_primary_cutoff_keeps_every_output(gpu, orc, case):
+def test_primary_cutoff_keeps_every_output(gpu, orc, case, request):
     """Without a step map (and outside the depth view) primary rays heading
     up stop once above every solid voxel, like shadow rays: every map the
     target has still equals the oracle's, the iteration total drops below the
     oracle's, and with a step map (or the cutoff off) nothing stops early."""
-    sc = gpu.make_scene("frame1080" if case == "frame1080_cameras" else "terrain128", seed=1 if case == "frame1080_cameras" else 4)
-    ctx = gpu.Context(sc.dims)
-    ctx.upload(sc.world)
-    ctx.build_depth_field()
-    wdf = ctx.download()
+    if case == "frame1080_cameras":
+        sc, ctx, wdf = request.getfixturevalue("frame1080")  # the oracle's own depth field
+        own = False
+    else:
+        sc = gpu.make_scene("terrain128", seed=4)
+        ctx = gpu.Context(sc.dims)
+        ctx.upload(sc.world)
+        ctx.build_depth_field()
+        wdf = orc.depth_field(sc.world, sc.dims)
+        own = True
     rng = np.random.default_rng(77)
     frames = [sc.frame]
     for _ in range(2):
@@ -305,7 +367,8 @@ def test_primary_cutoff_keeps_every_output(gpu, orc, case):
         assert res[False] <= res[True] <= r["dda_steps"]
         if k > 0:
             assert res[False] < res[True], "no primary ray stopped early in frame %d" % k
-    ctx.close()
+    if own:
+        ctx.close()
 
 
 def entering_rays(dims, n_per_face, rng):
