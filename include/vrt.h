@@ -272,6 +272,18 @@ int vrt_batch_launch(vrt_batch* b);
 int vrt_batch_destroy(vrt_batch* b);
 /* Renders each view with its own plain launch, one after another. */
 int vrt_draw_views_plain(vrt_ctx* ctx, const vrt_view* views, int32_t n);
+/* The views drawn like vrt_batch_create + vrt_batch_launch with the same
+ * config (cfg NULL = the context's tuned default), outputs bit-identical, but
+ * with no device allocation: the view descriptors and queue counters live in
+ * a fixed array of about 230 KiB per GPU inside the library, filled by
+ * kernels on the context's stream, so the call records only kernel nodes
+ * under stream capture.  At most 256 views per call (VRT_ERR_INVALID above).
+ * Calls that overlap on one GPU (other streams, other contexts) share that
+ * array and race; calls on one stream are ordered.  No counterpart in the
+ * reference: it is for callers that cannot allocate (HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc). */
+#define VRT_NOALLOC_MAX_VIEWS 256
+int vrt_draw_views_noalloc(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_batch_config* cfg);
 /* Times every tile shape in {8x8,16x4,32x2,16x16} x chunk size in
  * {0,1,2,4,8,16} on these views and keeps the fastest as the context's
  * default.  results (may be NULL) receives up to max_results entries. */

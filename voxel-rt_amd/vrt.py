@@ -125,6 +125,7 @@ SIGNATURES = {
     "vrt_batch_launch": (C.c_int, [P]),
     "vrt_batch_destroy": (C.c_int, [P]),
     "vrt_draw_views_plain": (C.c_int, [P, C.POINTER(vrt_view), I32]),
+    "vrt_draw_views_noalloc": (C.c_int, [P, C.POINTER(vrt_view), I32, C.POINTER(vrt_batch_config)]),
     "vrt_autotune_batch": (C.c_int, [P, C.POINTER(vrt_view), I32, I32, C.POINTER(vrt_batch_config),
                                      C.POINTER(vrt_tune_entry), I32, C.POINTER(I32)]),
     "vrt_server_create": (C.c_int, [P, C.POINTER(vrt_server_config), C.POINTER(P)]),
