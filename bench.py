@@ -358,6 +358,11 @@ def views64_profile(vrt, launches=3):
             "pixels": int(sum(w * h for (_, w, h) in views))}
 
 
+# served batches: up to 16 views and one 3840x2160 frame's pixels each, on 4 slots
+SERVED_BATCH_PIXELS = 3840 * 2160
+SERVED_SLOTS = 4
+
+
 def served_stream(vrt, ctx, params, dims, views, rate, edit_seed):
     """One request stream through a fresh server: requests arrive at Poisson
     times of `rate` per second (rate None: all at once, the saturation run),
@@ -371,7 +376,8 @@ def served_stream(vrt, ctx, params, dims, views, rate, edit_seed):
     cfg = vrt.vrt_server_config()
     cfg.framebuffer_budget = 32 << 30
     cfg.max_batch_views = 16
-    cfg.streams = 3
+    cfg.max_batch_pixels = SERVED_BATCH_PIXELS
+    cfg.streams = SERVED_SLOTS
     cfg.want_depth = 1
     cfg.want_aux = 0
     s = C.c_void_p()
@@ -447,8 +453,8 @@ def served_bench(vrt, n_requests=240, loads=(0.5, 0.9, 1.2, 2.0)):
     sat = served_stream(vrt, ctx, params, dims, views, None, 56)
     cap_rps = len(views) / sat["wall"]
     out = {"workload": "resident 1024x512x1024 grid (2 GiB), %d requests per stream, resolutions as the batched "
-                       "set, 10 removeSphere edits (r 1-8) per 100 requests, 32 GB framebuffer budget, 3 slots of up "
-                       "to 16 views" % n_requests,
+                       "set, 10 removeSphere edits (r 1-8) per 100 requests, 32 GB framebuffer budget, %d slots of up "
+                       "to 16 views and %d pixels" % (n_requests, SERVED_SLOTS, SERVED_BATCH_PIXELS),
            "saturation": served_summary(sat, "all at once")}
     out["capacity"] = {"requests_per_s": cap_rps, "mrays_s": sat["rays"] / sat["wall"] / 1e6,
                        "mrays_s_device_span": sat["rays"] / out["saturation"]["device_span_s"] / 1e6}

@@ -303,6 +303,10 @@ typedef struct vrt_server_config {
     int32_t want_depth;            /* allocate and fill depth per request */
     int32_t want_aux;              /* allocate and fill hit/face/shadow/steps maps */
     vrt_batch_config batch;        /* tile_w == 0: the context's default */
+    int64_t max_batch_pixels;      /* pixels per batch launch (0 = no limit); a batch always takes
+                                    * at least one request.  Every request of a batch completes when
+                                    * the whole batch does, so smaller batches on more slots cut the
+                                    * latency of small requests queued behind large ones */
 } vrt_server_config;
 
 typedef struct vrt_result {

@@ -163,7 +163,7 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
     ctx.upload(sc.world)
     ctx.build_depth_field()
     wdf = orc.depth_field(sc.world, sc.dims)  # the oracle replays the log on its own grid
-    views = small_views(gpu, sc, 14, seed=5)
+    views = small_views(gpu, sc, 15, seed=5)  # 5 edits: odd, so the server's own copy ends ahead
     rng = np.random.default_rng(6)
     s = C.c_void_p()
     gpu.call("vrt_server_create", ctx.h, C.byref(server_config(gpu, 1 << 30)), C.byref(s))
@@ -202,6 +202,15 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
     # the server's world is the oracle's after the whole log
     gpu.call("vrt_server_destroy", s)
     assert np.array_equal(ctx.download(), world)
+    # and the context's own draws see every edit once the server is gone (the
+    # server renders from two alternating copies of the grid)
+    f, w, h = views[0]
+    t = gpu.Target(w, h, depth=True, aux=True)
+    ctx.stats(reset=True)
+    ctx.draw(t, frame=f)
+    g = t.download()
+    g.update(ctx.stats(reset=True))
+    assert_frames_match(g, orc.render(world, sc.dims, f, w, h), "context draw after the server")
     ctx.close()
 
 

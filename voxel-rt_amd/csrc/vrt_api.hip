@@ -307,6 +307,11 @@ int refresh_render(vrt_ctx* c, int64_t x0, int64_t y0, int64_t z0, int64_t x1, i
 
 int refresh_render_on(vrt_ctx* c, hipStream_t s, int64_t x0, int64_t y0, int64_t z0, int64_t x1, int64_t y1,
                       int64_t z1) {
+    return refresh_copy_on(c, c->rgrid, s, x0, y0, z0, x1, y1, z1);
+}
+
+int refresh_copy_on(vrt_ctx* c, const GridDesc& dst, hipStream_t s, int64_t x0, int64_t y0, int64_t z0, int64_t x1,
+                    int64_t y1, int64_t z1) {
     x0 = std::max<int64_t>(x0, 0);
     y0 = std::max<int64_t>(y0, 0);
     z0 = std::max<int64_t>(z0, 0);
@@ -314,7 +319,7 @@ int refresh_render_on(vrt_ctx* c, hipStream_t s, int64_t x0, int64_t y0, int64_t
     y1 = std::min<int64_t>(y1, c->grid.Y);
     z1 = std::min<int64_t>(z1, c->grid.Z);
     if (x1 <= x0 || y1 <= y0 || z1 <= z0) return VRT_OK;
-    return launch_copy_box(c->grid, c->rgrid, x0, y0, z0, x1 - x0, y1 - y0, z1 - z0, s);
+    return launch_copy_box(c->grid, dst, x0, y0, z0, x1 - x0, y1 - y0, z1 - z0, s);
 }
 
 int remove_sphere_grid(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s) {
@@ -332,11 +337,16 @@ int remove_sphere_grid(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t r
 }
 
 int remove_sphere_refresh(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s) {
+    return remove_sphere_refresh_into(c, c->rgrid, cx, cy, cz, radius, s);
+}
+
+int remove_sphere_refresh_into(vrt_ctx* c, const GridDesc& dst, int32_t cx, int32_t cy, int32_t cz, int32_t radius,
+                               hipStream_t s) {
     // the carve covers [c - |r|, c + |r|) at most, the repair [c - R, c + R)
     const int64_t R = (int64_t)radius + (VRT_DEPTH_FIELD_RADIUS >> 1);
     const int64_t M = std::max<int64_t>(radius < 0 ? -(int64_t)radius : (int64_t)radius, R);
-    return refresh_render_on(c, s, (int64_t)cx - M, (int64_t)cy - M, (int64_t)cz - M, (int64_t)cx + M,
-                             (int64_t)cy + M, (int64_t)cz + M);
+    return refresh_copy_on(c, dst, s, (int64_t)cx - M, (int64_t)cy - M, (int64_t)cz - M, (int64_t)cx + M,
+                           (int64_t)cy + M, (int64_t)cz + M);
 }
 
 int refresh_render_all(vrt_ctx* c) { return refresh_render(c, 0, 0, 0, c->grid.X, c->grid.Y, c->grid.Z); }

@@ -221,6 +221,11 @@ int refresh_render_on(vrt_ctx* c, hipStream_t s, int64_t x0, int64_t y0, int64_t
 // into the render copy (what in-flight renders must not see).
 int remove_sphere_grid(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s);
 int remove_sphere_refresh(vrt_ctx* c, int32_t cx, int32_t cy, int32_t cz, int32_t radius, hipStream_t s);
+// The same two refreshes into another copy laid out like the render copy.
+int refresh_copy_on(vrt_ctx* c, const GridDesc& dst, hipStream_t s, int64_t x0, int64_t y0, int64_t z0, int64_t x1,
+                    int64_t y1, int64_t z1);
+int remove_sphere_refresh_into(vrt_ctx* c, const GridDesc& dst, int32_t cx, int32_t cy, int32_t cz, int32_t radius,
+                               hipStream_t s);
 }  // namespace vrt
 
 namespace vrt {

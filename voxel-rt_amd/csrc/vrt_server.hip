@@ -3,12 +3,16 @@
 // are admitted by framebuffer memory into continuous batches.
 //
 // Ordering.  Requests and edits form one FIFO.  A request sees every edit
-// queued before it and none after.  An edit runs on the edit stream in two
-// parts: the carve and depth-field repair of the grid start at once (render
-// kernels read only the padded render copy, never the grid), and the copy of
-// the edited box into the render copy waits (hipStreamWaitEvent) for every
-// batch in flight, which must not see it; every later batch waits for that
-// copy.  The host never blocks on an edit.
+// queued before it and none after.  Batches read one of two padded render
+// copies of the grid: the context's own and a second one the server keeps,
+// used in turn, version v on copy v mod 2.  Edit v runs on the edit stream:
+// the carve and depth-field repair of the grid start at once (render kernels
+// never read the grid), then the boxes of edits v-1 and v are copied from the
+// grid into copy v mod 2, which waits (hipStreamWaitEvent) only for the
+// batches still reading that copy, those of version v-2 or older; every later
+// batch waits for that copy.  The batches of version v-1 keep running on the
+// other copy meanwhile, so an edit does not drain the device.  The host never
+// blocks on an edit.
 //
 // Continuous batching.  Each pump admits queued requests, oldest first, while
 // their frame buffers fit in the budget and a batch slot (stream) is free, and
@@ -58,7 +62,13 @@ struct Item {  // FIFO entry: a request or an edit
     int32_t cx = 0, cy = 0, cz = 0, r = 0;
 };
 
+struct EditRec {
+    uint64_t version;
+    int32_t cx, cy, cz, r;
+};
+
 struct Slot {
+    int32_t copy = 0;  // render copy the batch reads
     hipStream_t stream = nullptr;
     hipEvent_t exec = nullptr;  // the batch reached the device (timing)
     hipEvent_t done = nullptr;  // the batch ended (timing)
@@ -124,6 +134,12 @@ struct vrt_server {
     uint64_t bytes_in_use = 0;
     int32_t queued = 0;
     Arena arena;
+    // copy[0] is the context's render copy, copy[1] the server's; each holds
+    // the grid as of copy_version[k]; `recent` the edits newer than the older copy
+    vrt::GridDesc copy[2];
+    uint64_t copy_version[2] = {0, 0};
+    int32_t* own_copy = nullptr;
+    std::deque<EditRec> recent;
 };
 
 namespace vrt {
@@ -210,17 +226,31 @@ int retire(vrt_server* s) {
     return VRT_OK;
 }
 
+// Brings render copy k up to version_applied: the boxes of the edits it has
+// not seen, copied from the grid on the edit stream after the batches that
+// read copy k.
+int catch_up(vrt_server* s, int32_t k) {
+    for (auto& sl : s->slots)
+        if (sl.busy && sl.copy == k) VRT_HIP(hipStreamWaitEvent(s->edit_stream, sl.done, 0));
+    for (const EditRec& e : s->recent)
+        if (e.version > s->copy_version[k])
+            if (int rc = remove_sphere_refresh_into(s->ctx, s->copy[k], e.cx, e.cy, e.cz, e.r, s->edit_stream))
+                return rc;
+    s->copy_version[k] = s->version_applied;
+    while (!s->recent.empty() && s->recent.front().version <= std::min(s->copy_version[0], s->copy_version[1]))
+        s->recent.pop_front();
+    return VRT_OK;
+}
+
 int apply_edit(vrt_server* s, const Item& it) {
     vrt_ctx* c = s->ctx;
     // carve + repair of the grid: no render reads it, so no wait
     if (int rc = remove_sphere_grid(c, it.cx, it.cy, it.cz, it.r, s->edit_stream)) return rc;
-    // the render copy changes only after every batch that must not see the edit
-    for (auto& sl : s->slots)
-        if (sl.busy) VRT_HIP(hipStreamWaitEvent(s->edit_stream, sl.done, 0));
-    if (int rc = remove_sphere_refresh(c, it.cx, it.cy, it.cz, it.r, s->edit_stream)) return rc;
+    s->version_applied++;
+    s->recent.push_back({s->version_applied, it.cx, it.cy, it.cz, it.r});
+    if (int rc = catch_up(s, (int32_t)(s->version_applied & 1))) return rc;
     VRT_HIP(hipEventRecord(s->edit_done, s->edit_stream));
     s->have_edit = true;
-    s->version_applied++;
     return VRT_OK;
 }
 
@@ -247,8 +277,11 @@ int launch_slot(vrt_server* s, Slot& sl, std::vector<uint64_t>& ids) {
                            sl.stream));
     if (cfg.chunk_tiles)
         if (int rc = launch_queue_reset(sl.queue, sl.stream)) return rc;
-    if (int rc = launch_render_batch(render_grid(c), sl.views_dev, (int32_t)descs.size(), total, cfg, wgs, sl.queue,
-                                     c->counters, sl.stream))
+    sl.copy = (int32_t)(s->version_applied & 1);
+    GridDesc g = render_grid(c);
+    g.vox = s->copy[sl.copy].vox;
+    if (int rc = launch_render_batch(g, sl.views_dev, (int32_t)descs.size(), total, cfg, wgs, sl.queue, c->counters,
+                                     sl.stream))
         return rc;
     VRT_HIP(hipEventRecord(sl.done, sl.stream));
     const double t = clock_of(s);
@@ -280,11 +313,15 @@ int admit(vrt_server* s, int* launched) {
             }
         if (!free_slot) return VRT_OK;
         std::vector<uint64_t> ids;
+        int64_t pixels = 0;
         while (!s->fifo.empty() && !s->fifo.front().is_edit && (int32_t)ids.size() < s->cfg.max_batch_views) {
             Request& r = s->reqs[s->fifo.front().id];
+            const int64_t px = (int64_t)r.width * r.height;
+            if (!ids.empty() && s->cfg.max_batch_pixels > 0 && pixels + px > s->cfg.max_batch_pixels) break;
             if (s->bytes_in_use + r.bytes > s->cfg.framebuffer_budget) break;
             if (!alloc_target(s, r)) break;  // no block large enough: wait for releases
             s->bytes_in_use += r.bytes;
+            pixels += px;
             ids.push_back(r.id);
             s->fifo.pop_front();
             s->queued--;
@@ -300,6 +337,9 @@ void destroy_server(vrt_server* s) {
     if (s->ctx) (void)hipSetDevice(s->ctx->device);
     for (auto& sl : s->slots)
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    for (auto& sl : s->slots) sl.busy = false;
+    // the context's render copy takes the edits only the server's copy has seen
+    if (s->edit_stream && s->own_copy && s->copy_version[0] != s->version_applied) (void)catch_up(s, 0);
     if (s->edit_stream) (void)hipStreamSynchronize(s->edit_stream);
     for (auto& sl : s->slots) {
         if (sl.exec) (void)hipEventDestroy(sl.exec);
@@ -310,6 +350,7 @@ void destroy_server(vrt_server* s) {
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     if (s->arena.base) (void)hipFree(s->arena.base);
+    if (s->own_copy) (void)hipFree(s->own_copy);
     if (s->edit_done) (void)hipEventDestroy(s->edit_done);
     if (s->ref) (void)hipEventDestroy(s->ref);
     if (s->edit_stream) (void)hipStreamDestroy(s->edit_stream);
@@ -327,7 +368,7 @@ int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** o
     if (!ctx || !ctx->grid.vox || !cfg || !out) return fail(VRT_ERR_INVALID, "invalid arguments");
     *out = nullptr;
     if (cfg->max_batch_views < 1 || cfg->max_batch_views > 4096 || cfg->streams < 1 || cfg->streams > 16 ||
-        cfg->framebuffer_budget == 0)
+        cfg->framebuffer_budget == 0 || cfg->max_batch_pixels < 0)
         return fail(VRT_ERR_INVALID, "server config out of range");
     // tile_w == 0 takes the context's default (already checked when tuned)
     if (cfg->batch.tile_w != 0)
@@ -358,6 +399,15 @@ int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** o
     if ((e = hipStreamWaitEvent(s->edit_stream, s->edit_done, 0)) != hipSuccess)
         return bail(hip_fail(e, "hipStreamWaitEvent"));
     s->have_edit = true;
+    // the second render copy, equal to the context's
+    s->copy[0] = ctx->rgrid;
+    s->copy[1] = ctx->rgrid;
+    if ((e = hipMalloc(&s->own_copy, ctx->rgrid.count * sizeof(int32_t))) != hipSuccess) {
+        s->own_copy = nullptr;
+        return bail(hip_fail(e, "hipMalloc of the second render copy"));
+    }
+    s->copy[1].vox = s->own_copy;
+    if (int rc = launch_copy(s->own_copy, ctx->rgrid.vox, ctx->rgrid.count, s->edit_stream)) return bail(rc);
     // the clock anchor: an event the host waits for, stamped with the host time
     if ((e = hipEventRecord(s->ref, s->edit_stream)) != hipSuccess) return bail(hip_fail(e, "hipEventRecord"));
     if ((e = hipEventSynchronize(s->ref)) != hipSuccess) return bail(hip_fail(e, "hipEventSynchronize"));

@@ -63,7 +63,8 @@ class vrt_tune_entry(C.Structure):
 
 class vrt_server_config(C.Structure):
     _fields_ = [("framebuffer_budget", C.c_uint64), ("max_batch_views", C.c_int32), ("streams", C.c_int32),
-                ("want_depth", C.c_int32), ("want_aux", C.c_int32), ("batch", vrt_batch_config)]
+                ("want_depth", C.c_int32), ("want_aux", C.c_int32), ("batch", vrt_batch_config),
+                ("max_batch_pixels", C.c_int64)]
 
 
 class vrt_result(C.Structure):
