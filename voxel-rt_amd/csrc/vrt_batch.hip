@@ -94,8 +94,9 @@ int build_view_descs(const vrt_view* views, int32_t n, const vrt_batch_config& c
 }
 
 int persistent_workgroups(vrt_ctx* c, const vrt_batch_config& cfg, int64_t total_tiles, int32_t* out) {
-    if (cfg.chunk_tiles == 0) {  // hardware-scheduled: one workgroup per four tiles, no queue
-        *out = (int32_t)std::min<int64_t>((total_tiles + 3) / 4, 0x7fffffff);
+    if (cfg.chunk_tiles == 0) {  // hardware-scheduled: one wave per 64 pixels, four per workgroup, no queue
+        const int64_t items = total_tiles * (cfg.tile_w * cfg.tile_h / 64);
+        *out = (int32_t)std::min<int64_t>((items + 3) / 4, 0x7fffffff);
         return VRT_OK;
     }
     int32_t waves = cfg.waves_per_cu;

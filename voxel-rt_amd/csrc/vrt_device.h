@@ -350,7 +350,7 @@ __device__ __forceinline__ void cast_ray(const GridDesc& g, float sx, float sy, 
 struct PixelResult {
     uint8_t rgb[3];
     float depth;
-    int64_t hit;
+    int64_t hit_word;  // word index of the hit in the padded render copy, -1 on a miss
     uint8_t face;
     uint32_t eval, occ;
     uint32_t steps;
@@ -392,7 +392,7 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
     const float ny = h.axis == 1 ? (float)(-h.step) : 0.0f;
     const float nz = h.axis == 2 ? (float)(-h.step) : 0.0f;
 
-    o.hit = hit ? IX::ref_index(g, h.index) : -1;
+    o.hit_word = hit ? h.index : -1;
     o.face = hit ? (uint8_t)(h.axis * 3 + (h.step + 1)) : (uint8_t)VRT_FACE_MISS;
     o.steps = h.iters;
     if (hit) {
@@ -499,13 +499,16 @@ __device__ __forceinline__ void shade_pixel(const GridDesc& g, const FrameDesc& 
     o.iters = iters;
 }
 
-__device__ __forceinline__ void store_pixel(const vrt_target& t, int64_t k, const PixelResult& p) {
+// The hit's reference index x + X*y + X*Y*z (two integer divisions) is formed
+// only when the target has a hit map.
+template <typename IX>
+__device__ __forceinline__ void store_pixel(const GridDesc& g, const vrt_target& t, int64_t k, const PixelResult& p) {
     uint8_t* c = t.rgb + 3 * k;
     c[0] = p.rgb[0];
     c[1] = p.rgb[1];
     c[2] = p.rgb[2];
     if (t.depth) t.depth[k] = p.depth;
-    if (t.hit) t.hit[k] = p.hit;
+    if (t.hit) t.hit[k] = p.hit_word != -1 ? IX::ref_index(g, p.hit_word) : -1;
     if (t.face) t.face[k] = p.face;
     if (t.shadow_eval) t.shadow_eval[k] = p.eval;
     if (t.shadow_occ) t.shadow_occ[k] = p.occ;

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "vrt_device.h"
 #include "vrt_internal.h"
@@ -52,6 +53,31 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
     }
 }
 
+// The same for one tile per wave, with wave sums in 32 bits: a lane casts at
+// most 18 rays (primary, sun, 16 lights) of at most `lim` iterations each (a
+// walk ends at its budget), so 64 * 18 * lim < 2^32 keeps every sum exact;
+// rays (at most 1152) and pixels (at most 64) share one word.
+__device__ __forceinline__ void flush_counters_tile(unsigned long long* counters, uint32_t rays, uint32_t iters,
+                                                    uint32_t pixels, float lim) {
+    if (!counters) return;
+    if (!(lim < 3.7e6f)) {
+        flush_counters(counters, rays, iters, pixels);
+        return;
+    }
+    uint32_t rp = rays | (pixels << 16);
+    for (int o = 32; o > 0; o >>= 1) {
+        rp += (uint32_t)__shfl_xor((int)rp, o, 64);
+        iters += (uint32_t)__shfl_xor((int)iters, o, 64);
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        const uint32_t wave = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+        unsigned long long* c = counters + (size_t)(wave % kCounterSlots) * kCounterStride;
+        atomicAdd(c + 0, (unsigned long long)(rp & 0xffffu));
+        atomicAdd(c + 1, (unsigned long long)iters);
+        atomicAdd(c + 2, (unsigned long long)(rp >> 16));
+    }
+}
+
 // waves_per_eu(8, 8): without it the build ran at 7 resident waves per SIMD
 // (peak measured with per-wave timestamps: 7168 waves on 1024 SIMDs); with it,
 // 8 (a few registers of the rare IEEE-division path spill), 4% faster.
@@ -65,7 +91,7 @@ k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counte
     // the wave's tile is uniform: the tile arithmetic runs on the scalar unit
     const int32_t tile = (int32_t)lb * kPlainWaves + __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
     const int32_t lane = threadIdx.x & 63;
-    unsigned long long rays = 0, iters = 0, pixels = 0;
+    uint32_t rays = 0, iters = 0, pixels = 0;
     if (tile < tiles_x * tiles_y) {
         int32_t trow = tile / tiles_x + F.row_rot;
         if (trow >= tiles_y) trow -= tiles_y;
@@ -76,13 +102,13 @@ k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counte
             dev::pixel_vpos(t.width, t.height, F.inv_w, F.inv_h, px, row, vx, vy);
             dev::PixelResult p;
             dev::shade_pixel<IX>(g, F, vx, vy, p, top);
-            dev::store_pixel(t, (int64_t)row * t.width + px, p);
+            dev::store_pixel<IX>(g, t, (int64_t)row * t.width + px, p);
             rays = p.rays;
             iters = p.iters;
             pixels = 1;
         }
     }
-    flush_counters(counters, rays, iters, pixels);
+    flush_counters_tile(counters, rays, iters, pixels, F.render_lim);
 }
 
 template <typename IX>
@@ -112,20 +138,22 @@ __global__ void __launch_bounds__(256) k_cast_rays(GridDesc g, int64_t n, const 
 }
 
 // One tile of a multi-view launch: `tile` indexes the concatenated tile lists
-// of the views (largest view first, each row-major).  Tile shape tw x th with
-// tw*th a multiple of 64: the wave shades 64 pixels per pass (lane ->
-// (lane % tw, lane / tw)) and tw*th/64 passes.  Per-lane totals stay far
-// below 2^32 for any launch the host accepts (tiles per launch < 2^32; 17
-// rays and at most 17 * 2^24 steps a pixel would overflow only past ~15
-// pixels at the step cap); they are widened when the wave flushes.
-template <typename IX>
+// of the views (largest view first, each row-major).  Tile shape TW x TH with
+// TW a power of two and TW*TH a multiple of 64: the wave shades 64 pixels per
+// pass (lane -> (lane % TW, lane / TW)) and TW*TH/64 passes.  The shape is a
+// template parameter so that the pass loop has a constant trip count and
+// constant lane arithmetic (with runtime shapes the kernels spilled registers
+// to scratch).  Per-lane totals stay far below 2^32 for any launch the host
+// accepts (tiles per launch < 2^32; 18 rays and at most 18 * 2^24 steps a
+// pixel would overflow only past ~14 pixels at the step cap); they are widened
+// when the wave flushes.
+template <typename IX, int TW, int TH>
 __device__ __forceinline__ void shade_view_tile(const GridDesc& g, const ViewDesc* __restrict__ views,
-                                                int32_t n_views, int64_t tile, int32_t tw, int32_t th,
-                                                int32_t lane, int32_t top, uint32_t& rays, uint32_t& iters,
-                                                uint32_t& pixels) {
-    const int32_t tw_log2 = __builtin_ctz((uint32_t)tw);  // tw is a power of two
-    const int32_t rows_per_pass = kWave >> tw_log2;
-    const int32_t passes = th / rows_per_pass;
+                                                int32_t n_views, int64_t tile, int32_t first_pass,
+                                                int32_t n_passes, int32_t lane, int32_t top, uint32_t& rays,
+                                                uint32_t& iters, uint32_t& pixels) {
+    static_assert((TW & (TW - 1)) == 0 && TW <= kWave && (TW * TH) % kWave == 0, "tile shape");
+    constexpr int32_t kRowsPerPass = kWave / TW;
     // view owning this tile (uniform across the wave)
     int32_t lo = 0, hi = n_views - 1;
     while (lo < hi) {
@@ -135,17 +163,17 @@ __device__ __forceinline__ void shade_view_tile(const GridDesc& g, const ViewDes
     }
     const ViewDesc& V = views[__builtin_amdgcn_readfirstlane(lo)];
     const int32_t local = (int32_t)(tile - V.first_tile);
-    const int32_t tx0 = (local % V.tiles_x) * tw;
-    const int32_t ty0 = (local / V.tiles_x) * th;
-    for (int32_t pass = 0; pass < passes; pass++) {
-        const int32_t px = tx0 + (lane & (tw - 1));
-        const int32_t row = ty0 + pass * rows_per_pass + (lane >> tw_log2);
+    const int32_t tx0 = (local % V.tiles_x) * TW;
+    const int32_t ty0 = (local / V.tiles_x) * TH;
+    const int32_t px = tx0 + (lane & (TW - 1));
+    for (int32_t pass = first_pass; pass < first_pass + n_passes; pass++) {
+        const int32_t row = ty0 + pass * kRowsPerPass + lane / TW;
         if (px < V.target.width && row < V.target.height) {
             float vx, vy;
             dev::pixel_vpos(V.target.width, V.target.height, V.frame.inv_w, V.frame.inv_h, px, row, vx, vy);
             dev::PixelResult p;
             dev::shade_pixel<IX>(g, V.frame, vx, vy, p, top);
-            dev::store_pixel(V.target, (int64_t)row * V.target.width + px, p);
+            dev::store_pixel<IX>(g, V.target, (int64_t)row * V.target.width + px, p);
             rays += p.rays;
             iters += p.iters;
             pixels += 1;
@@ -153,19 +181,26 @@ __device__ __forceinline__ void shade_view_tile(const GridDesc& g, const ViewDes
     }
 }
 
-// Multi-view launch scheduled by the hardware: a 256-thread workgroup takes
-// four consecutive tiles (one per wave, sharing the CU's L1), and the
-// dispatcher hands out workgroups in order as slots free up, so the views'
-// tiles are balanced over the device without any queue.
-template <typename IX>
+// Multi-view launch scheduled by the hardware.  Every wave shades one pass of
+// 64 pixels: work item i is pass i mod P of tile i / P (P = TW*TH/64), and a
+// 256-thread workgroup takes four consecutive items, so its waves share the
+// CU's L1 over one 16x16 tile or four adjacent smaller ones.  With one pass
+// per wave no value lives across a second walk, which keeps every tile shape
+// within 64 VGPRs without spilling.  The dispatcher hands out workgroups in
+// order as slots free up, so the views' tiles are balanced over the device
+// without any queue.
+template <typename IX, int TW, int TH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k_render_views(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t tw,
-               int32_t th, unsigned long long* counters) {
+k_render_views(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles,
+               unsigned long long* counters) {
+    constexpr int32_t kPasses = TW * TH / kWave;
     const int32_t top = g.top ? *g.top : INT32_MAX;
     const int32_t lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
+    const int64_t item = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int32_t)(threadIdx.x >> 6));
     uint32_t rays = 0, iters = 0, pixels = 0;
-    if (tile < total_tiles) shade_view_tile<IX>(g, views, n_views, tile, tw, th, lane, top, rays, iters, pixels);
+    if (item < total_tiles * kPasses)
+        shade_view_tile<IX, TW, TH>(g, views, n_views, item / kPasses, (int32_t)(item % kPasses), 1, lane, top, rays,
+                                    iters, pixels);
     flush_counters(counters, rays, iters, pixels);
 }
 
@@ -177,26 +212,31 @@ k_render_views(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, 
 // device-scope atomics on one word per XCD, so they pay off only when tiles
 // are large: on one 1080p frame of 8x8 tiles this kernel ran 50% slower than
 // k_render_views (2.8x slower with a single counter for all XCDs).
-template <typename IX>
+// Held to 8 waves per SIMD, the queue loop's state spills about 28 SGPRs
+// into VGPR lanes and one VGPR to scratch; at the compiler's own budget (67
+// VGPRs, 7 waves, no spill) the 64-view set took 11.7 ms instead of 10.4.
+template <typename IX, int TW, int TH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t tw,
-               int32_t th, int32_t chunk, unsigned int* queue, unsigned long long* counters) {
+k_render_batch(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles, int32_t chunk,
+               unsigned int* queue, unsigned long long* counters) {
+    constexpr uint32_t kPasses = TW * TH / kWave;
     const int32_t top = g.top ? *g.top : INT32_MAX;
     const int32_t lane = threadIdx.x & 63;
     uint32_t rays = 0, iters = 0, pixels = 0;
+    const uint32_t total = (uint32_t)total_tiles;  // below 2^32 (host check)
     const uint32_t home = blockIdx.x % kXcds;
     for (uint32_t k = 0; k < kXcds; k++) {
         const uint32_t q = (home + k) % kXcds;
-        const int64_t r0 = total_tiles * q / kXcds, r1 = total_tiles * (q + 1) / kXcds;
+        const uint32_t r0 = (uint32_t)((uint64_t)total * q / kXcds), r1 = (uint32_t)((uint64_t)total * (q + 1) / kXcds);
         for (;;) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(queue + q * kQueueStride, (unsigned int)chunk);
             base = __builtin_amdgcn_readfirstlane(base);
-            if ((int64_t)base >= r1 - r0) break;
-            int64_t end = r0 + (int64_t)base + chunk;
-            if (end > r1) end = r1;
-            for (int64_t tile = r0 + base; tile < end; tile++)
-                shade_view_tile<IX>(g, views, n_views, tile, tw, th, lane, top, rays, iters, pixels);
+            if (base >= r1 - r0) break;
+            const uint32_t end = r1 - r0 - base > (uint32_t)chunk ? r0 + base + (uint32_t)chunk : r1;
+            for (uint64_t item = (uint64_t)(r0 + base) * kPasses; item < (uint64_t)end * kPasses; item++)
+                shade_view_tile<IX, TW, TH>(g, views, n_views, (int64_t)(item / kPasses), (int32_t)(item % kPasses), 1,
+                                            lane, top, rays, iters, pixels);
         }
     }
     flush_counters(counters, rays, iters, pixels);
@@ -278,10 +318,22 @@ int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const flo
     return VRT_OK;
 }
 
+// Calls f with the tile shape as template arguments (check_batch_config's set).
+template <typename F>
+int with_shape(int32_t tw, int32_t th, F&& f) {
+    if (tw == 8 && th == 8) f(std::integral_constant<int, 8>{}, std::integral_constant<int, 8>{});
+    else if (tw == 16 && th == 4) f(std::integral_constant<int, 16>{}, std::integral_constant<int, 4>{});
+    else if (tw == 32 && th == 2) f(std::integral_constant<int, 32>{}, std::integral_constant<int, 2>{});
+    else if (tw == 16 && th == 16) f(std::integral_constant<int, 16>{}, std::integral_constant<int, 16>{});
+    else if (tw == 64 && th == 1) f(std::integral_constant<int, 64>{}, std::integral_constant<int, 1>{});
+    else return fail(VRT_ERR_INVALID, "unsupported tile shape");
+    return VRT_OK;
+}
+
 int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu) {
     (void)cfg;
     int blocks = 0;
-    VRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_batch<dev::Idx64>, 256, 0));
+    VRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_render_batch<dev::Idx64, 8, 8>, 256, 0));
     *waves_per_cu = blocks * 4;
     return VRT_OK;
 }
@@ -290,21 +342,29 @@ int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_
                         const vrt_batch_config& cfg, int32_t n_workgroups, unsigned int* queue,
                         unsigned long long* counters, hipStream_t s) {
     if (cfg.chunk_tiles == 0) {
-        const int64_t wgs = (total_tiles + 3) / 4;
+        const int64_t wgs = (total_tiles * (cfg.tile_w * cfg.tile_h / 64) + 3) / 4;  // one wave per 64 pixels
         if (wgs <= 0) return VRT_OK;
         if (wgs > 0x7fffffff) return fail(VRT_ERR_INVALID, "too many tiles in one batch");
-        with_index(g, [&](auto ix) {
-            hipLaunchKernelGGL(k_render_views<decltype(ix)>, dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev,
-                               n_views, total_tiles, cfg.tile_w, cfg.tile_h, counters);
-        });
+        if (int rc = with_shape(cfg.tile_w, cfg.tile_h, [&](auto tw, auto th) {
+                with_index(g, [&](auto ix) {
+                    hipLaunchKernelGGL((k_render_views<decltype(ix), decltype(tw)::value, decltype(th)::value>),
+                                       dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev, n_views, total_tiles,
+                                       counters);
+                });
+            }))
+            return rc;
         VRT_HIP(hipGetLastError());
         return VRT_OK;
     }
     if (n_workgroups <= 0) return fail(VRT_ERR_INVALID, "no workgroups");
-    with_index(g, [&](auto ix) {
-        hipLaunchKernelGGL(k_render_batch<decltype(ix)>, dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev,
-                           n_views, total_tiles, cfg.tile_w, cfg.tile_h, cfg.chunk_tiles, queue, counters);
-    });
+    if (int rc = with_shape(cfg.tile_w, cfg.tile_h, [&](auto tw, auto th) {
+            with_index(g, [&](auto ix) {
+                hipLaunchKernelGGL((k_render_batch<decltype(ix), decltype(tw)::value, decltype(th)::value>),
+                                   dim3((uint32_t)n_workgroups), dim3(256), 0, s, g, views_dev, n_views, total_tiles,
+                                   cfg.chunk_tiles, queue, counters);
+            });
+        }))
+        return rc;
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
