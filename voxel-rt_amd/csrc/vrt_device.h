@@ -524,11 +524,29 @@ __device__ __forceinline__ void pixel_vpos(int32_t W, int32_t H, float inv_w, fl
     vy = div_small_int((float)(2 * py + 1), (float)H, inv_h) - 1.0f;
 }
 
-// Sum over the wave, result in every lane.
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// Sum of a 32-bit value over a full wave (all 64 lanes active), on the scalar
+// unit: prefix sums within each row of 16 lanes by DPP row shifts (zeros
+// shifted in), then the row totals carried on by row broadcasts, so lane 63
+// holds the total.  Six VALU adds instead of the lane-index arithmetic and
+// LDS permutes of a shuffle tree.  The caller keeps the total below 2^32.
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, true);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
+
+// The same for per-lane values below 2^58: the low 26 bits and the rest are
+// summed apart (64 lanes of 26 bits stay below 2^32), then recombined.
+__device__ __forceinline__ unsigned long long wave_sum_u58(unsigned long long v) {
+    const uint32_t lo = wave_sum_u32((uint32_t)(v & 0x3ffffffull));
+    const uint32_t hi = wave_sum_u32((uint32_t)(v >> 26));
+    return (unsigned long long)lo + ((unsigned long long)hi << 26);
+}
+
 
 }  // namespace dev
 }  // namespace vrt

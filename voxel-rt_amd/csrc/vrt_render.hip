@@ -38,12 +38,14 @@ constexpr int kXcds = 8;
 constexpr int kQueueStride = 32;  // queue counters 128 bytes apart
 
 // Adds this wave's totals into its counter slot (see kCounterSlots).
+// Every lane of the wave must be active.  Per-lane totals are 32-bit sums of
+// per-pixel counts, far below 2^58.
 __device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long rays,
                                                unsigned long long iters, unsigned long long pixels) {
     if (!counters) return;
-    rays = dev::wave_sum(rays);
-    iters = dev::wave_sum(iters);
-    pixels = dev::wave_sum(pixels);
+    rays = dev::wave_sum_u58(rays);
+    iters = dev::wave_sum_u58(iters);
+    pixels = dev::wave_sum_u58(pixels);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         const uint32_t wave = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
         unsigned long long* c = counters + (size_t)(wave % kCounterSlots) * kCounterStride;
@@ -64,11 +66,8 @@ __device__ __forceinline__ void flush_counters_tile(unsigned long long* counters
         flush_counters(counters, rays, iters, pixels);
         return;
     }
-    uint32_t rp = rays | (pixels << 16);
-    for (int o = 32; o > 0; o >>= 1) {
-        rp += (uint32_t)__shfl_xor((int)rp, o, 64);
-        iters += (uint32_t)__shfl_xor((int)iters, o, 64);
-    }
+    const uint32_t rp = dev::wave_sum_u32(rays | (pixels << 16));
+    iters = dev::wave_sum_u32(iters);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         const uint32_t wave = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
         unsigned long long* c = counters + (size_t)(wave % kCounterSlots) * kCounterStride;
