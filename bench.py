@@ -68,7 +68,7 @@ def profiled(key):
     FETCH_SIZE + WRITE_SIZE), "valu_lane_ops_per_launch"
     (SQ_THREAD_CYCLES_VALU, one per active lane per VALU instruction) or
     "valu_issue_frac" (4 x SQ_INSTS_VALU over 1024 SIMDs x the launch's
-    cycles: the share of VALU issue slots used).  None when the summary was
+    cycles).  None when the summary was
     taken on other kernel sources (its kernel_source_sha16 line differs from
     the current sources' hash): a stale figure is not reported."""
     vals = {}
@@ -152,8 +152,7 @@ def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
     if lane_ops:
         valu_achieved = lane_ops / (kernel_ms * 1e-3) / 1e12
         valu = {"achieved": valu_achieved, "peak": VALU_PEAK_T, "unit": "T lane-ops/s",
-                "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops,
-                "issue_frac_profiled": profiled("valu_issue_frac"), "limiter": True}
+                "frac": valu_achieved / VALU_PEAK_T, "lane_ops_per_launch": lane_ops, "limiter": True}
     # The same frames with the cutoff off: every output is identical,
     # the DDA walks are the reference's full length (its iteration count).
     full = None
@@ -168,7 +167,8 @@ def plain_bench(vrt, steps, warmup, cutoff_off_leg=True):
                      "bytes_per_launch": int(bytes_per_launch), "kernel": "k_render_plain",
                      "note": "achieved = algorithmic bytes (4 B per DDA iteration + 7 B per pixel) over the "
                              "kernel time; the measured HBM traffic is below them (the grid is served from L2) "
-                             "and the kernel is bound by VALU issue (valu)", "valu": valu},
+                             "and the kernel is bound by VALU throughput (valu: lane-ops against 16 lanes per SIMD "
+                             "per cycle)", "valu": valu},
         "shadow_cutoff_off": full,
     }
     return sc, ctx, out
