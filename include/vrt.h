@@ -325,6 +325,12 @@ typedef struct vrt_result {
 
 typedef struct vrt_server vrt_server;
 
+/* The server takes over the context until vrt_server_destroy: its edits and
+ * batches run on streams of its own, ordered after whatever the context had
+ * queued at creation but not with later work on the context's stream, so
+ * draws, edits or uploads made on the context meanwhile race the server's.
+ * vrt_server_destroy waits for the server's work and leaves the context's
+ * grid and render copy with every served edit applied. */
 int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** out);
 int vrt_server_destroy(vrt_server* s);
 /* Queue a render request; it sees every edit submitted before it and none

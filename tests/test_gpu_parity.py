@@ -266,6 +266,15 @@ def test_region_upload_with_float_corners(gpu):
     got = run((X - 2.0, -1.0, 0.0), (X + 1.0, 2.0, 1.0))
     cells = [(X - 2, 0, 0), (X - 1, 0, 0), (0, 1, 0), (1, 1, 0), (X - 2, 1, 0), (X - 1, 1, 0), (0, 2, 0), (1, 2, 0)]
     assert np.array_equal(got, expect(cells))
+    # whole rows of whole slabs (runs that meet end to end): z = 1, 2 entirely
+    got = run((0.0, 0.0, 1.0), (X - 1.0, Y - 0.5, 3.0))
+    assert np.array_equal(got, expect([(x, y, z) for z in (1, 2) for y in range(Y) for x in range(X)]))
+    # both corners outside (no swap); runs of int(2X - (X - 2)) + 1 = X + 3 words from
+    # x = X - 2 of rows 0 and 1 (j = -1 skipped), the second starting inside the first:
+    # cells X - 2 .. 3X of slab 0 in row-major order
+    got = run((X - 2.0, -1.0, 0.0), (2.0 * X, 2.0, 1.0))
+    cells = [(k % X, k // X, 0) for k in range(X - 2, 3 * X + 1)]
+    assert np.array_equal(got, expect(cells))
     # corners whose float loops would not end are refused
     ctx = gpu.Context((X, Y, Z))
     w = np.ascontiguousarray(host)

@@ -155,6 +155,27 @@ int32_t plain_row_rotation(const vrt_frame& f, const vrt_target& t, int32_t top,
     return r == tiles_y ? 0 : r;
 }
 
+namespace {
+// True when work on `s` is being captured into a graph (or the state cannot
+// be read: then the caller takes the path that is right under capture).
+bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+        (void)hipGetLastError();
+        return true;
+    }
+    return st != hipStreamCaptureStatusNone;
+}
+
+// Adds the plain draws' pending partial counts into the counters.
+int fold_parts(vrt_ctx* c) {
+    if (c->parts_used <= 0) return VRT_OK;
+    if (int rc = launch_fold_parts(c->parts, c->parts_used, c->counters, c->stream)) return rc;
+    c->parts_used = 0;
+    return VRT_OK;
+}
+}  // namespace
+
 int check_frame(const vrt_frame& f) {
     if (f.render_dist < 0 || f.render_dist > (1 << 24))
         return fail(VRT_ERR_INVALID, "render_dist must be in [0, 2^24]");
@@ -240,6 +261,7 @@ void release(vrt_ctx* c) {
     if (c->rgrid.vox) (void)hipFree(c->rgrid.vox);
     if (c->grid.top) (void)hipFree(c->grid.top);
     if (c->counters) (void)hipFree(c->counters);
+    if (c->parts) (void)hipFree(c->parts);
     if (c->table_packed) (void)hipFree(c->table_packed);
     if (c->table_dist) (void)hipFree(c->table_dist);
     if (c->table_rows) (void)hipFree(c->table_rows);
@@ -426,6 +448,8 @@ int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device) {
     const size_t counter_bytes = (size_t)kCounterSlots * kCounterStride * sizeof(unsigned long long);
     if ((e = hipMalloc(&c->counters, counter_bytes)) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc counters"));
+    if ((e = hipMalloc(&c->parts, (size_t)kPartsCap * sizeof(unsigned long long))) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc partial counts"));
     const uint64_t nbricks = (uint64_t)((X + 7) / 8) * ((Y + 7) / 8) * ((Z + 7) / 8);
     if ((e = hipMalloc(&c->bricks, nbricks)) != hipSuccess) return bail(hip_fail(e, "hipMalloc bricks"));
     int rc = upload_table(c);
@@ -594,16 +618,31 @@ int vrt_upload_region_f(vrt_ctx* ctx, const int32_t* host_voxels, const float st
     const std::vector<int64_t> zs = rows(s0[2], e0[2], g.Z), ys = rows(s0[1], e0[1], g.Y);
     const int64_t xs = (int32_t)s0[0];
     if (xs < 0 || xs >= (int64_t)g.X || zs.empty() || ys.empty()) return VRT_OK;
-    int64_t lo = INT64_MAX, hi = -1;
+    // Runs in increasing offset order (z outer, y inner, both ascending);
+    // runs that touch or overlap are copied as one range: they carry the same
+    // host words to the same cells.
+    int64_t lo = INT64_MAX, hi = -1, run_lo = -1, run_hi = -1;
+    auto flush = [&]() -> int {
+        if (run_hi < 0) return VRT_OK;
+        VRT_HIP(hipMemcpyAsync(g.vox + run_lo, host_voxels + run_lo, (size_t)(run_hi - run_lo) * sizeof(int32_t),
+                               hipMemcpyHostToDevice, ctx->stream));
+        return VRT_OK;
+    };
     for (int64_t z : zs)
         for (int64_t y : ys) {
             const int64_t off = xs + (int64_t)g.X * y + (int64_t)g.XY * z;
             if ((uint64_t)(off + x_len) > g.count) continue;
-            VRT_HIP(hipMemcpyAsync(g.vox + off, host_voxels + off, (size_t)x_len * sizeof(int32_t),
-                                   hipMemcpyHostToDevice, ctx->stream));
+            if (run_hi >= 0 && off <= run_hi) {
+                run_hi = std::max(run_hi, off + x_len);
+            } else {
+                if (int rc = flush()) return rc;
+                run_lo = off;
+                run_hi = off + x_len;
+            }
             lo = std::min(lo, off);
             hi = std::max(hi, off + x_len);
         }
+    if (int rc = flush()) return rc;
     if (hi < 0) return VRT_OK;
     // the cells written lie in whole x-rows of the z-slabs lo..hi-1 touch
     const int64_t z_lo = lo / (int64_t)g.XY, z_hi = (hi - 1) / (int64_t)g.XY + 1;
@@ -788,7 +827,19 @@ int vrt_draw_frame(vrt_ctx* ctx, const vrt_frame* f, const vrt_target* t) {
     ctx->draws++;
     FrameDesc F = make_frame_desc(*f, *t);
     if (ctx->shadow_cutoff) F.row_rot = plain_row_rotation(*f, *t, ctx->top_host, F.primary_cutoff != 0);
-    return launch_render_plain(render_grid(ctx), F, *t, ctx->counters, ctx->stream);
+    // the tile totals go into the partial-count array (kPartsCap) unless the
+    // draw is being captured, the frame has more tiles than the array or
+    // its 32-bit tile sums could overflow
+    const int64_t tiles = (int64_t)((t->width + 7) / 8) * ((t->height + 7) / 8);
+    unsigned long long* parts = nullptr;
+    if (F.render_lim < kTileSum32Lim && tiles <= kPartsCap && !stream_capturing(ctx->stream)) {
+        if (ctx->parts_used + tiles > kPartsCap)
+            if (int rc = fold_parts(ctx)) return rc;
+        parts = ctx->parts + ctx->parts_used;
+    }
+    if (int rc = launch_render_plain(render_grid(ctx), F, *t, ctx->counters, parts, ctx->stream)) return rc;
+    if (parts) ctx->parts_used += tiles;
+    return VRT_OK;
 }
 
 // glDrawArrays over the fullscreen quad, src/main.cpp:59.
@@ -799,6 +850,7 @@ int vrt_draw(vrt_ctx* ctx, const vrt_target* t) {
 
 int vrt_read_stats(vrt_ctx* ctx, vrt_stats* out, int32_t reset) {
     if (int rc = check_ctx(ctx)) return rc;
+    if (int rc = fold_parts(ctx)) return rc;
     std::vector<unsigned long long> h((size_t)kCounterSlots * kCounterStride);
     const size_t bytes = h.size() * sizeof(unsigned long long);
     VRT_HIP(hipMemcpyAsync(h.data(), ctx->counters, bytes, hipMemcpyDeviceToHost, ctx->stream));

@@ -90,6 +90,20 @@ struct FrameDesc {
 constexpr int kCounterSlots = 4096;
 constexpr int kCounterStride = 16;  // u64 per slot
 
+// Plain draws do not add into the counters at the end of every wave: a
+// device-scope atomic there keeps the wave's slot (and with it the start of
+// the next workgroup, which needs a slot on every SIMD of the CU) until it
+// completes, 3% of a 1080p frame.  Each wave stores its tile's totals into
+// the context's partial-count array instead, one u64 per tile (DDA iterations
+// in the low 32 bits, rays and pixels in 16 bits each above them), at the
+// next free entries; the entries are added into the counters by a fold kernel
+// when the array is full and before stats are read.  Under stream capture
+// the draw keeps the atomics (a replayed graph would rewrite one set of
+// entries).  The packed sums are exact while a tile's iteration total stays
+// below 2^32: 64 lanes x 18 rays x lim < 2^32 for lim < kTileSum32Lim.
+constexpr int64_t kPartsCap = (int64_t)1 << 22;  // entries (32 MiB)
+constexpr float kTileSum32Lim = 3.7e6f;
+
 // Queue of the persistent batch kernel: 8 counters (one per XCD), 128 B apart.
 constexpr size_t kQueueBytes = 8 * 128;
 // Tiles one multi-view launch may hold (queue counters are 32-bit).
@@ -120,8 +134,12 @@ struct DepthTableDev {
 struct Ctx;
 
 // ---- kernels launch wrappers (vrt_render.hip / vrt_edit.hip / vrt_world.hip)
+// parts == nullptr: the wave totals go into `counters` by atomics; else into
+// parts[tile] (see kPartsCap).
 int launch_render_plain(const GridDesc& g, const FrameDesc& f, const vrt_target& t,
-                        unsigned long long* counters, hipStream_t s);
+                        unsigned long long* counters, unsigned long long* parts, hipStream_t s);
+// Adds parts[0, n) into the counter slots.
+int launch_fold_parts(const unsigned long long* parts, int64_t n, unsigned long long* counters, hipStream_t s);
 int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const float* dir,
                      const int32_t* dist, int32_t render_dist, int64_t* hit, float* hit_pos,
                      uint8_t* face, uint32_t* steps, hipStream_t s);
@@ -184,6 +202,8 @@ struct vrt_ctx {
     vrt_frame frame{};
     bool frame_set = false;
     unsigned long long* counters = nullptr;  // rays, iters, pixels
+    unsigned long long* parts = nullptr;     // plain draws' partial counts (kPartsCap entries)
+    int64_t parts_used = 0;                  // entries not yet folded into `counters`
     uint64_t draws = 0;
     int32_t* table_packed = nullptr;
     float* table_dist = nullptr;

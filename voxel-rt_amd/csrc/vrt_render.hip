@@ -62,7 +62,7 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
 __device__ __forceinline__ void flush_counters_tile(unsigned long long* counters, uint32_t rays, uint32_t iters,
                                                     uint32_t pixels, float lim) {
     if (!counters) return;
-    if (!(lim < 3.7e6f)) {
+    if (!(lim < kTileSum32Lim)) {
         flush_counters(counters, rays, iters, pixels);
         return;
     }
@@ -82,7 +82,7 @@ __device__ __forceinline__ void flush_counters_tile(unsigned long long* counters
 // 8 (a few registers of the rare IEEE-division path spill), 4% faster.
 template <typename IX>
 __global__ void __launch_bounds__(kPlainThreads) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counters) {
+k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counters, unsigned long long* parts) {
     const int32_t top = g.top ? *g.top : INT32_MAX;
     const uint32_t lb = blockIdx.x;
     const int32_t tiles_x = (t.width + 7) >> 3;
@@ -107,7 +107,24 @@ k_render_plain(GridDesc g, FrameDesc F, vrt_target t, unsigned long long* counte
             pixels = 1;
         }
     }
-    flush_counters_tile(counters, rays, iters, pixels, F.render_lim);
+    if (parts) {
+        // the tile's totals by plain store (the host passes parts only when
+        // F.render_lim < kTileSum32Lim, so the 32-bit sums are exact)
+        const uint32_t rp = dev::wave_sum_u32(rays | (pixels << 16));
+        const uint32_t it = dev::wave_sum_u32(iters);
+        if (lane == 0 && tile < tiles_x * tiles_y) parts[tile] = (unsigned long long)it | ((unsigned long long)rp << 32);
+    } else {
+        flush_counters_tile(counters, rays, iters, pixels, F.render_lim);
+    }
+}
+
+// Adds plain draws' partial counts (iterations | rays << 32 | pixels << 48
+// per tile) into the counter slots; every lane of the grid takes part.
+__global__ void __launch_bounds__(256) k_fold_parts(const unsigned long long* __restrict__ parts, int64_t n,
+                                                     unsigned long long* counters) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const unsigned long long v = i < n ? parts[i] : 0ull;
+    flush_counters(counters, (v >> 32) & 0xffffull, v & 0xffffffffull, v >> 48);
 }
 
 template <typename IX>
@@ -290,15 +307,24 @@ void with_index(const GridDesc& g, F&& f) {
 }  // namespace
 
 int launch_render_plain(const GridDesc& g, const FrameDesc& f, const vrt_target& t, unsigned long long* counters,
-                        hipStream_t s) {
+                        unsigned long long* parts, hipStream_t s) {
     const int64_t tiles = (int64_t)((t.width + 7) / 8) * ((t.height + 7) / 8);
     const int64_t blocks = (tiles + kPlainWaves - 1) / kPlainWaves;
     if (blocks <= 0) return VRT_OK;
     if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "frame too large");
     with_index(g, [&](auto ix) {
         hipLaunchKernelGGL(k_render_plain<decltype(ix)>, dim3((uint32_t)blocks), dim3(kPlainThreads), 0, s, g, f, t,
-                           counters);
+                           counters, parts);
     });
+    VRT_HIP(hipGetLastError());
+    return VRT_OK;
+}
+
+int launch_fold_parts(const unsigned long long* parts, int64_t n, unsigned long long* counters, hipStream_t s) {
+    if (n <= 0) return VRT_OK;
+    const int64_t blocks = (n + 255) / 256;
+    if (blocks > 0x7fffffff) return fail(VRT_ERR_INVALID, "too many partial counts");
+    hipLaunchKernelGGL(k_fold_parts, dim3((uint32_t)blocks), dim3(256), 0, s, parts, n, counters);
     VRT_HIP(hipGetLastError());
     return VRT_OK;
 }
