@@ -2,7 +2,8 @@
 // launch.  A plan uploads the views' descriptors once (largest view first, so
 // the long-running tiles start early and small views fill the tail), then
 // captures the launch into a hipGraph that every vrt_batch_launch replays:
-// either the hardware-scheduled kernel (chunk_tiles 0), or "reset the tile
+// either "the hardware-scheduled kernel, then the fold of its per-pass
+// counts into the context's counters" (chunk_tiles 0), or "reset the tile
 // queue; launch the persistent kernel" (the reset is a one-thread kernel, not
 // a memset node).  The autotuner times every tile shape and chunk size and
 // keeps the fastest for the context.
@@ -21,6 +22,8 @@ struct vrt_batch {
     std::vector<vrt::ViewDesc> views;
     vrt::ViewDesc* views_dev = nullptr;
     unsigned int* queue = nullptr;
+    unsigned long long* parts = nullptr;  // per-pass counts (hardware-scheduled plans)
+    int64_t items = -1;                   // entries of `parts`, -1: counts by atomics
     int64_t total_tiles = 0;
     int32_t workgroups = 0;
     hipStream_t capture = nullptr;
@@ -29,6 +32,13 @@ struct vrt_batch {
 };
 
 namespace vrt {
+
+int64_t batch_items(const std::vector<ViewDesc>& views, int64_t total_tiles, const vrt_batch_config& cfg) {
+    if (cfg.chunk_tiles != 0) return -1;
+    for (const ViewDesc& v : views)
+        if (!(v.frame.render_lim < kTileSum32Lim)) return -1;
+    return total_tiles * (cfg.tile_w * cfg.tile_h / 64);
+}
 
 int check_batch_config(const vrt_batch_config& c) {
     const bool shape_ok = (c.tile_w == 8 && c.tile_h == 8) || (c.tile_w == 16 && c.tile_h == 4) ||
@@ -51,6 +61,7 @@ void destroy_plan(vrt_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->views_dev) (void)hipFree(b->views_dev);
     if (b->queue) (void)hipFree(b->queue);
+    if (b->parts) (void)hipFree(b->parts);
     if (b->capture) (void)hipStreamDestroy(b->capture);
     delete b;
 }
@@ -138,6 +149,11 @@ int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_b
     if ((e = hipMalloc(&b->views_dev, b->views.size() * sizeof(ViewDesc))) != hipSuccess)
         return bail(hip_fail(e, "hipMalloc views"));
     if ((e = hipMalloc(&b->queue, kQueueBytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc queue"));
+    // the hardware-scheduled kernel's per-pass counts, up to 2^26 entries (512 MiB)
+    b->items = batch_items(b->views, b->total_tiles, cfg);
+    if (b->items > ((int64_t)1 << 26)) b->items = -1;
+    if (b->items > 0 && (e = hipMalloc(&b->parts, (size_t)b->items * sizeof(unsigned long long))) != hipSuccess)
+        return bail(hip_fail(e, "hipMalloc partial counts"));
     if ((e = hipMemcpy(b->views_dev, b->views.data(), b->views.size() * sizeof(ViewDesc), hipMemcpyHostToDevice)) !=
         hipSuccess)
         return bail(hip_fail(e, "hipMemcpy views"));
@@ -147,8 +163,9 @@ int vrt_batch_create(vrt_ctx* ctx, const vrt_view* views, int32_t n, const vrt_b
         return bail(hip_fail(e, "hipStreamBeginCapture"));
     int rc = cfg.chunk_tiles ? launch_queue_reset(b->queue, b->capture) : VRT_OK;
     if (!rc)
-        rc = launch_render_batch(render_grid(ctx), b->views_dev, (int32_t)b->views.size(), b->total_tiles, cfg, b->workgroups,
-                                 b->queue, ctx->counters, b->capture);
+        rc = launch_render_batch(render_grid(ctx), b->views_dev, (int32_t)b->views.size(), b->total_tiles, cfg,
+                                 b->workgroups, b->queue, ctx->counters, b->parts, b->capture);
+    if (!rc && b->parts) rc = launch_fold_parts(b->parts, b->items, ctx->counters, b->capture);
     e = hipStreamEndCapture(b->capture, &b->graph);
     if (rc) return bail(rc);
     if (e != hipSuccess) return bail(hip_fail(e, "hipStreamEndCapture"));

@@ -143,9 +143,16 @@ int launch_fold_parts(const unsigned long long* parts, int64_t n, unsigned long 
 int launch_cast_rays(const GridDesc& g, int64_t n, const float* start, const float* dir,
                      const int32_t* dist, int32_t render_dist, int64_t* hit, float* hit_pos,
                      uint8_t* face, uint32_t* steps, hipStream_t s);
+// parts (chunk_tiles 0 only; nullptr: atomics): one entry per 64-pixel pass
+// (batch_items), added into the counters by launch_fold_parts afterwards.
 int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_views,
                         int64_t total_tiles, const vrt_batch_config& cfg, int32_t n_workgroups,
-                        unsigned int* queue, unsigned long long* counters, hipStream_t s);
+                        unsigned int* queue, unsigned long long* counters, unsigned long long* parts,
+                        hipStream_t s);
+// Entries a hardware-scheduled multi-view launch writes into `parts`, or -1
+// when its counts must go by atomics (a view's render_lim too large for the
+// 32-bit pass sums, or a persistent configuration).
+int64_t batch_items(const std::vector<ViewDesc>& views, int64_t total_tiles, const vrt_batch_config& cfg);
 int launch_queue_reset(unsigned int* queue, hipStream_t s);
 int selftest_division(uint32_t d_begin, uint32_t d_count, unsigned long long* bad_dev, hipStream_t s);
 int selftest_reciprocal(uint64_t first, uint64_t count, unsigned long long* bad_dev, hipStream_t s);

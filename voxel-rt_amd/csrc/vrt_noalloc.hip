@@ -116,7 +116,7 @@ int vrt_draw_views_noalloc(vrt_ctx* ctx, const vrt_view* views, int32_t n, const
     if (cfg.chunk_tiles)
         if (int rc = launch_queue_reset(queue, ctx->stream)) return rc;  // a kernel, not a memset
     if (int rc = launch_render_batch(render_grid(ctx), views_dev, n, total_tiles, cfg, wgs, queue, ctx->counters,
-                                     ctx->stream))
+                                     nullptr, ctx->stream))
         return rc;
     ctx->draws += (uint64_t)n;
     return VRT_OK;

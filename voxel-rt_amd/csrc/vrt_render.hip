@@ -208,7 +208,7 @@ __device__ __forceinline__ void shade_view_tile(const GridDesc& g, const ViewDes
 template <typename IX, int TW, int TH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 k_render_views(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, int64_t total_tiles,
-               unsigned long long* counters) {
+               unsigned long long* counters, unsigned long long* parts) {
     constexpr int32_t kPasses = TW * TH / kWave;
     const int32_t top = g.top ? *g.top : INT32_MAX;
     const int32_t lane = threadIdx.x & 63;
@@ -217,7 +217,16 @@ k_render_views(GridDesc g, const ViewDesc* __restrict__ views, int32_t n_views, 
     if (item < total_tiles * kPasses)
         shade_view_tile<IX, TW, TH>(g, views, n_views, item / kPasses, (int32_t)(item % kPasses), 1, lane, top, rays,
                                     iters, pixels);
-    flush_counters(counters, rays, iters, pixels);
+    if (parts) {
+        // one pass of 64 pixels has a tile's bounds; the host passes parts
+        // only when every view's render_lim < kTileSum32Lim
+        const uint32_t rp = dev::wave_sum_u32(rays | (pixels << 16));
+        const uint32_t it = dev::wave_sum_u32(iters);
+        if (lane == 0 && item < total_tiles * kPasses)
+            parts[item] = (unsigned long long)it | ((unsigned long long)rp << 32);
+    } else {
+        flush_counters(counters, rays, iters, pixels);
+    }
 }
 
 // Persistent multi-view kernel.  The tiles are split into kXcds contiguous
@@ -365,7 +374,8 @@ int batch_occupancy(const vrt_batch_config& cfg, int32_t* waves_per_cu) {
 
 int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_views, int64_t total_tiles,
                         const vrt_batch_config& cfg, int32_t n_workgroups, unsigned int* queue,
-                        unsigned long long* counters, hipStream_t s) {
+                        unsigned long long* counters, unsigned long long* parts, hipStream_t s) {
+    if (parts && cfg.chunk_tiles != 0) return fail(VRT_ERR_INVALID, "partial counts need the hardware-scheduled kernel");
     if (cfg.chunk_tiles == 0) {
         const int64_t wgs = (total_tiles * (cfg.tile_w * cfg.tile_h / 64) + 3) / 4;  // one wave per 64 pixels
         if (wgs <= 0) return VRT_OK;
@@ -374,7 +384,7 @@ int launch_render_batch(const GridDesc& g, const ViewDesc* views_dev, int32_t n_
                 with_index(g, [&](auto ix) {
                     hipLaunchKernelGGL((k_render_views<decltype(ix), decltype(tw)::value, decltype(th)::value>),
                                        dim3((uint32_t)wgs), dim3(256), 0, s, g, views_dev, n_views, total_tiles,
-                                       counters);
+                                       counters, parts);
                 });
             }))
             return rc;

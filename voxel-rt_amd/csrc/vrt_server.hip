@@ -56,6 +56,10 @@ struct Request {
     bool done = false;
 };
 
+// Per-pass count entries a slot keeps: a batch whose hardware-scheduled
+// launch has more passes adds its counts by atomics instead.
+constexpr int64_t kSlotParts = (int64_t)1 << 20;
+
 struct Item {  // FIFO entry: a request or an edit
     bool is_edit = false;
     uint64_t id = 0;  // request id
@@ -75,6 +79,7 @@ struct Slot {
     vrt::ViewDesc* views_dev = nullptr;
     vrt::ViewDesc* views_host = nullptr;  // pinned staging
     unsigned int* queue = nullptr;
+    unsigned long long* parts = nullptr;  // per-pass counts of the slot's batch (kSlotParts entries)
     bool busy = false;
     std::vector<uint64_t> ids;
 };
@@ -280,9 +285,13 @@ int launch_slot(vrt_server* s, Slot& sl, std::vector<uint64_t>& ids) {
     sl.copy = (int32_t)(s->version_applied & 1);
     GridDesc g = render_grid(c);
     g.vox = s->copy[sl.copy].vox;
+    const int64_t items = batch_items(descs, total, cfg);
+    unsigned long long* parts = items > 0 && items <= kSlotParts ? sl.parts : nullptr;
     if (int rc = launch_render_batch(g, sl.views_dev, (int32_t)descs.size(), total, cfg, wgs, sl.queue, c->counters,
-                                     sl.stream))
+                                     parts, sl.stream))
         return rc;
+    if (parts)
+        if (int rc = launch_fold_parts(parts, items, c->counters, sl.stream)) return rc;
     VRT_HIP(hipEventRecord(sl.done, sl.stream));
     const double t = clock_of(s);
     for (uint64_t id : ids) {
@@ -347,6 +356,7 @@ void destroy_server(vrt_server* s) {
         if (sl.views_dev) (void)hipFree(sl.views_dev);
         if (sl.views_host) (void)hipHostFree(sl.views_host);
         if (sl.queue) (void)hipFree(sl.queue);
+        if (sl.parts) (void)hipFree(sl.parts);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     if (s->arena.base) (void)hipFree(s->arena.base);
@@ -423,6 +433,8 @@ int vrt_server_create(vrt_ctx* ctx, const vrt_server_config* cfg, vrt_server** o
         if ((e = hipHostMalloc(&sl.views_host, vbytes, hipHostMallocDefault)) != hipSuccess)
             return bail(hip_fail(e, "hipHostMalloc"));
         if ((e = hipMalloc(&sl.queue, kQueueBytes)) != hipSuccess) return bail(hip_fail(e, "hipMalloc"));
+        if ((e = hipMalloc(&sl.parts, (size_t)kSlotParts * sizeof(unsigned long long))) != hipSuccess)
+            return bail(hip_fail(e, "hipMalloc"));
     }
     *out = s;
     return VRT_OK;
