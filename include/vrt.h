@@ -113,7 +113,9 @@ int vrt_device_count(int32_t* count);
  * initRender does before initVoxels (src/render.cpp:349-351). */
 int vrt_create(vrt_ctx** out, int64_t X, int64_t Y, int64_t Z, int32_t device);
 int vrt_destroy(vrt_ctx* ctx);
-/* hipStream_t to order all work of this context on (NULL = the context's own stream). */
+/* hipStream_t to order all work of this context on (NULL = the context's own
+ * stream).  Work queued before the switch stays ahead of work queued after it
+ * (the new stream waits for an event recorded on the old one). */
 int vrt_set_stream(vrt_ctx* ctx, void* stream);
 void* vrt_get_stream(vrt_ctx* ctx);
 int vrt_sync(vrt_ctx* ctx);

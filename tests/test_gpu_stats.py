@@ -94,3 +94,36 @@ def test_plain_draw_captured_in_a_graph(gpu):
     gpu.hip_check(h.hipGraphExecDestroy(exe), "hipGraphExecDestroy")
     gpu.hip_check(h.hipGraphDestroy(graph), "hipGraphDestroy")
     ctx.close()
+
+
+def test_stats_follow_a_stream_switch(gpu):
+    """Draws queued on the context's own stream, then more on a caller's
+    stream after vrt_set_stream: the totals read on the new stream count all
+    of them (the new stream waits for the old one's work, folds included)."""
+    h = gpu.hip()
+    h.hipStreamCreate.argtypes = [C.POINTER(C.c_void_p)]
+    h.hipStreamDestroy.argtypes = [C.c_void_p]
+    h.hipStreamSynchronize.argtypes = [C.c_void_p]
+    sc = gpu.make_scene("terrain128", seed=1)
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    t = gpu.Target(sc.width, sc.height, depth=False, aux=False)
+    ctx.stats(reset=True)
+    ctx.draw(t, frame=sc.frame)
+    one = ctx.stats(reset=True)
+    user = C.c_void_p()
+    gpu.hip_check(h.hipStreamCreate(C.byref(user)), "hipStreamCreate")
+    for _ in range(5):
+        ctx.draw(t, frame=sc.frame)
+    gpu.call("vrt_set_stream", ctx.h, user)
+    assert ctx.stream == user.value
+    for _ in range(3):
+        ctx.draw(t, frame=sc.frame)
+    st = ctx.stats(reset=True)
+    for k in ("rays", "dda_steps", "pixels"):
+        assert st[k] == 8 * one[k], (k, st[k], one[k])
+    gpu.call("vrt_set_stream", ctx.h, None)
+    gpu.hip_check(h.hipStreamSynchronize(user), "hipStreamSynchronize")
+    ctx.close()
+    gpu.hip_check(h.hipStreamDestroy(user), "hipStreamDestroy")

@@ -477,7 +477,17 @@ int vrt_destroy(vrt_ctx* ctx) {
 
 int vrt_set_stream(vrt_ctx* ctx, void* stream) {
     if (int rc = check_ctx(ctx)) return rc;
-    ctx->stream = stream ? (hipStream_t)stream : ctx->own_stream;
+    const hipStream_t next = stream ? (hipStream_t)stream : ctx->own_stream;
+    if (next == ctx->stream) return VRT_OK;
+    // work queued on the old stream (uploads, edits, draws whose counts are
+    // not folded yet) stays ahead of everything queued on the new one
+    hipEvent_t e = nullptr;
+    VRT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    hipError_t err = hipEventRecord(e, ctx->stream);
+    if (err == hipSuccess) err = hipStreamWaitEvent(next, e, 0);
+    (void)hipEventDestroy(e);
+    if (err != hipSuccess) return hip_fail(err, "ordering the new stream after the old one");
+    ctx->stream = next;
     return VRT_OK;
 }
 
