@@ -165,6 +165,7 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
     wdf = orc.depth_field(sc.world, sc.dims)  # the oracle replays the log on its own grid
     views = small_views(gpu, sc, 15, seed=5)  # 5 edits: odd, so the server's own copy ends ahead
     rng = np.random.default_rng(6)
+    ctx.stats(reset=True)
     s = C.c_void_p()
     gpu.call("vrt_server_create", ctx.h, C.byref(server_config(gpu, 1 << 30)), C.byref(s))
     log = []       # ("req", id, view) or ("edit", (cx,cy,cz,r))
@@ -183,11 +184,13 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
             gpu.call("vrt_server_pump", s, 0)  # admit part of the stream while the rest arrives
     got = drain(gpu, s, len(views))
     assert len(got) == len(views)
+    served = ctx.stats(reset=True)  # every batch (and its count fold) has ended
     for rid, (_, _, r) in got.items():  # device times sit between the host's (1 ms for the clock anchor)
         assert r.t_start <= r.t_exec + 1e-3 and r.t_exec <= r.t_gpu_done and r.t_gpu_done <= r.t_done + 1e-3, rid
     # the oracle replays the same log
     world = wdf.copy()
     version = 0
+    rays = steps = pixels = 0
     for entry in log:
         if entry[0] == "edit":
             orc.remove_sphere(world, sc.dims, entry[1][:3], entry[1][3])
@@ -199,6 +202,9 @@ def test_server_frames_equal_oracle_replaying_the_edit_log(gpu, orc):
         r = orc.render(world, sc.dims, f, w, h)
         out["rays"], out["dda_steps"] = r["rays"], r["dda_steps"]
         assert_frames_match(out, r, "served request %d" % rid)
+        rays, steps, pixels = rays + r["rays"], steps + r["dda_steps"], pixels + w * h
+    # the served batches' statistics: every ray counted once, no iteration more than the oracle's
+    assert served["rays"] == rays and served["pixels"] == pixels and 0 < served["dda_steps"] <= steps
     # the server's world is the oracle's after the whole log
     gpu.call("vrt_server_destroy", s)
     assert np.array_equal(ctx.download(), world)
