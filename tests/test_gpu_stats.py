@@ -127,3 +127,40 @@ def test_stats_follow_a_stream_switch(gpu):
     gpu.hip_check(h.hipStreamSynchronize(user), "hipStreamSynchronize")
     ctx.close()
     gpu.hip_check(h.hipStreamDestroy(user), "hipStreamDestroy")
+
+
+def test_stats_with_a_render_distance_beyond_the_packed_sums(gpu, orc):
+    """render_dist 4,000,000 >= the packed sums' bound (64 x 18 x lim < 2^32):
+    the plain draw and a hardware-scheduled batch add their counts by atomics
+    instead, and the totals still equal the oracle's."""
+    sc = gpu.make_scene("tiny", seed=2)
+    sc.frame.render_dist = 4_000_000
+    ctx = gpu.Context(sc.dims)
+    ctx.upload(sc.world)
+    ctx.build_depth_field()
+    ctx.set_shadow_cutoff(False)
+    w, h = 96, 64
+    r = orc.render(orc.depth_field(sc.world, sc.dims), sc.dims, sc.frame, w, h)
+    t = gpu.Target(w, h, depth=True, aux=True)
+    ctx.stats(reset=True)
+    ctx.draw(t, frame=sc.frame)
+    g = t.download()
+    g.update(ctx.stats(reset=True))
+    assert g["rays"] == r["rays"] and g["dda_steps"] == r["dda_steps"] and g["pixels"] == w * h
+    assert np.array_equal(g["hit"], r["hit"]) and np.array_equal(g["steps"], r["steps"])
+    # the same frame twice in one hardware-scheduled batch (8x8 tiles, chunk 0)
+    arr = (gpu.vrt_view * 2)()
+    targets = [gpu.Target(w, h, depth=False, aux=False) for _ in range(2)]
+    for i in range(2):
+        arr[i].frame = sc.frame
+        arr[i].target = targets[i].struct()
+    cfg = gpu.vrt_batch_config()
+    cfg.tile_w, cfg.tile_h, cfg.chunk_tiles = 8, 8, 0
+    b = C.c_void_p()
+    gpu.call("vrt_batch_create", ctx.h, arr, 2, C.byref(cfg), C.byref(b))
+    ctx.stats(reset=True)
+    gpu.call("vrt_batch_launch", b)
+    st = ctx.stats(reset=True)
+    gpu.call("vrt_batch_destroy", b)
+    assert st["rays"] == 2 * r["rays"] and st["dda_steps"] == 2 * r["dda_steps"]
+    ctx.close()
